@@ -1,0 +1,229 @@
+"""Benchmark of the TSQR hot path on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+
+Workload (default "tsqr_max_fp32"): FP32 TSQR at the maximum height the
+reference supports, m = 23,887,872, n = 32 (A is 3.06 GB), iid uniform[-1,1)
+entries from a seeded generator, resident in HBM before timing starts.  One
+step is one full tsqr_s_stream call (A overwritten by the explicit Q, R
+written).  The kernels have no data-dependent control flow, so step k > 1
+factors the Q left by step k-1 with exactly the same work; A is not
+re-uploaded between steps.
+
+metric  TSQR HBM GB/s = B_alg / (seconds per step), with
+        B_alg = (3mn + n^2) * 4 bytes (A read twice and written once are the
+        lower bound for an explicit Q that does not fit on chip).
+roofline  for the dominant kernel (tsqr_leafq: reads A and writes Q,
+        2mn*4 algorithmic bytes per launch) from HIP events on the launch
+        stream over the timed steps, against the 8 TB/s HBM peak.
+cpu_baseline  the oracle's C restatement of the reference algorithm
+        (OpenMP over tiles, all host cores) on a bounded sample of the same
+        workload (m/8 rows), plus OpenBLAS sgeqrf+sorgqr on the same sample.
+
+Other configurations (27648x32, 27648x128, FP64, batched, mixed) are
+exercised by the parity tests; --workload selects them for ad-hoc runs.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (kind, m, n, dtype)
+    "tsqr_max_fp32": ("tsqr", 23887872, 32, "float32"),
+    "tsqr_max_fp64": ("tsqr", 5971968, 32, "float64"),
+    "tsqr_27648x32_fp32": ("tsqr", 27648, 32, "float32"),
+    "qr_27648x128_fp32": ("qr", 27648, 128, "float32"),
+    "qr_27648x128_fp64": ("qr", 27648, 128, "float64"),
+}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--workload", default="tsqr_max_fp32", choices=sorted(WORKLOADS))
+    p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    p.add_argument("--cpu-fraction", type=int, default=8, help="cpu sample = m / this")
+    return p.parse_args()
+
+
+def cpu_baseline(m: int, n: int, dtype: str, frac: int) -> dict:
+    """Times the oracle restatement and LAPACK on a bounded host sample."""
+    import numpy as np
+
+    from oracle import oracle as O  # the cpu_baseline leg is the only bench user of the oracle
+
+    ms = max(n, (m // frac) // 27648 * 27648) if m >= 27648 * frac else m
+    A = O.gen_uniform(ms, n, np.dtype(dtype))
+    cores = os.cpu_count() or 1
+    esz = np.dtype(dtype).itemsize
+    balg = (3 * ms * n + n * n) * esz
+    t0 = time.perf_counter()
+    O.tsqr_ref(A, bs=256 if dtype == "float32" else 128, num_sm=108)
+    t_port = time.perf_counter() - t0
+    out = {
+        "value": round(balg / t_port / 1e9, 3),
+        "unit": "GB/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{ms}x{n} {dtype} (1/{frac} of the rows), oracle restatement of the reference "
+                  f"tsqr (NUM_SM=108, BLOCK_SIZE={256 if dtype == 'float32' else 128}), OpenMP over tiles",
+        "seconds": round(t_port, 3),
+    }
+    try:
+        import scipy.linalg.lapack as la
+        from threadpoolctl import threadpool_limits
+
+        with threadpool_limits(limits=cores):
+            Af = np.asfortranarray(A)
+            geqrf = la.sgeqrf if dtype == "float32" else la.dgeqrf
+            orgqr = la.sorgqr if dtype == "float32" else la.dorgqr
+            t0 = time.perf_counter()
+            qr_, tau, _, _ = geqrf(Af, overwrite_a=True)
+            orgqr(qr_, tau, overwrite_a=True)
+            t_lapack = time.perf_counter() - t0
+        out["lapack"] = {
+            "value": round(balg / t_lapack / 1e9, 3),
+            "unit": "GB/s",
+            "seconds": round(t_lapack, 3),
+            "what": f"OpenBLAS {'s' if dtype == 'float32' else 'd'}geqrf+{'s' if dtype == 'float32' else 'd'}orgqr "
+                    f"on the same sample, {cores} threads",
+        }
+    except Exception as e:  # LAPACK is a reported extra, not required
+        out["lapack"] = {"error": str(e)}
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; the row-sharded multi-GPU TSQR "
+                                   "(one R all-gather per call) is designed but not built"}))
+        sys.exit(2)
+
+    import torch
+
+    import tsqr_amd
+    from tsqr_amd._lib import TSQR_DOUBLE, TSQR_FLOAT, timing_read
+
+    kind, m, n, dtype = WORKLOADS[args.workload]
+    tdt = torch.float32 if dtype == "float32" else torch.float64
+    esz = 4 if dtype == "float32" else 8
+    code = TSQR_FLOAT if dtype == "float32" else TSQR_DOUBLE
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+
+    print(f"# bench: {args.workload} {kind} m={m} n={n} {dtype}", file=sys.stderr, flush=True)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    A = torch.empty((n, m), dtype=tdt, device=dev)
+    A.uniform_(-1.0, 1.0, generator=g)  # column-major m x n with lda = m
+    R = torch.zeros((n, n), dtype=tdt, device=dev)
+    L = tsqr_amd.lib()
+    need = (L.tsqr_workspace_bytes if kind == "tsqr" else L.qr_workspace_bytes)(m, n, code)
+    work = torch.empty(need, dtype=torch.uint8, device=dev)
+    fn = getattr(L, f"{kind}_{'s' if code == TSQR_FLOAT else 'd'}_stream")
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        st = fn(m, n, A.data_ptr(), m, R.data_ptr(), n, work.data_ptr(), need, stream.cuda_stream)
+        if st != 0:
+            raise tsqr_amd.TsqrError(st, "bench step")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    L.tsqr_timing_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    phases = timing_read()
+    L.tsqr_timing_enable(0)
+    print(f"# phases (ms total, count): {phases}", file=sys.stderr, flush=True)
+
+    sec = (t1 - t0) / args.steps
+    balg = (3 * m * n + n * n) * esz if kind == "tsqr" else None
+    if kind == "qr":
+        # n = 32*2^k: every column block is read twice and written once by its
+        # panel TSQR, plus the coupling products' traffic; report the panel rate
+        balg = (3 * m * n + n * n) * esz
+    value = balg / sec / 1e9
+
+    # Algorithmic bytes per launch of the two streaming kernels: the leaf R
+    # pass reads A (m*n elements), the leaf Q pass reads A and writes Q.
+    panel_n = n if kind == "tsqr" else 32
+    alg = {"tsqr_leaf_r": m * panel_n * esz, "tsqr_leafq": 2 * m * panel_n * esz}
+    kern = {}
+    for name, nbytes in alg.items():
+        tot_ms, cnt = phases[name]
+        if cnt:
+            avg_s = tot_ms / cnt / 1e3
+            kern[name] = {"avg_launch_ms": round(avg_s * 1e3, 4), "algorithmic_bytes_per_launch": nbytes,
+                          "achieved_GBs": round(nbytes / avg_s / 1e9, 1)}
+    roofline = None
+    if kern:
+        dom = max(kern, key=lambda k: kern[k]["avg_launch_ms"])
+        achieved = kern[dom]["achieved_GBs"]
+        roofline = {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "avg_launch_ms": kern[dom]["avg_launch_ms"],
+            "algorithmic_bytes_per_launch": kern[dom]["algorithmic_bytes_per_launch"],
+            "kernels": kern,
+        }
+    phase_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in phases.items() if v[1]}
+
+    out = {
+        "metric": "TSQR HBM GB/s (% of peak); batched QR/s mixed vs FP32 at equal error",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if dtype == "float32" else "f64",
+        "data": "synthetic: iid uniform[-1,1) from torch.Generator seed 0, resident in HBM",
+        "config": {
+            "workload": f"{kind} m={m} n={n} {dtype} (TSQR at the reference's maximum height)"
+            if args.workload == "tsqr_max_fp32" else f"{kind} m={m} n={n} {dtype}",
+            "m": m,
+            "n": n,
+            "algorithmic_bytes_per_step": balg,
+            "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
+            "phase_ms": phase_ms,
+        },
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(m, n, dtype, args.cpu_fraction)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,110 @@
+/*
+ * tsqr_amd.h -- C ABI of the MI355X (gfx950) TSQR / recursive blocked QR
+ * library (libtsqr_amd.so).
+ *
+ * Every entry point replaces one template of the reference library
+ * dajunHuang/tsqr, which is header-only C++ with no extern "C" symbols:
+ *
+ *   template <typename T>
+ *   void tsqr(int m, int n, T *A, int lda, T *R, int ldr, T *d_work,
+ *             int ldwork);                         // TallShinnyQR.h:86-87
+ *   template <typename T>
+ *   void qr(int m, int n, T *A, int lda, T *R, int ldr, T *d_work,
+ *           int ldwork);                           // block_QR.h:72-73
+ *
+ * instantiated for float and double (TallShinnyQR.h:150-153,
+ * block_QR.h:96-99).  Semantics kept from the reference:
+ *   - A, R and the work array are device pointers owned by the caller;
+ *   - A is m x n, column-major, leading dimension lda, and is overwritten by
+ *     the explicit thin Q (same lda);
+ *   - R is n x n, column-major, leading dimension ldr, written in full with
+ *     zeros below the diagonal (the reference's qr leaves the lower-left
+ *     block unwritten; here it is zeroed);
+ *   - the call is asynchronous with respect to the host; the plain entry
+ *     points run on the null stream, like the reference's launches.
+ * Differences (documented, deliberate):
+ *   - every call returns a status code instead of assert()/printf()/throw
+ *     (TallShinnyQR.h:92-103, utils.h:47-55);
+ *   - the accepted domain is m >= n >= 1 for any m (the reference needs
+ *     n | m and caps m at (NUM_SM*BS/n)*NUM_SM*BS, TallShinnyQR.h:100);
+ *     tsqr_* takes n <= 32 (the reference's effective limit, kernelQR.h:7);
+ *     qr_* takes any n <= m;
+ *   - the work array of the reference-shaped entry points holds ldwork*n
+ *     elements; if that is smaller than tsqr_workspace_bytes(), the library
+ *     uses a per-device internal buffer instead (not safe for concurrent
+ *     calls on one device -- pass a large enough work array for that);
+ *   - a zero column yields a zero R diagonal entry and an identity reflector
+ *     (LAPACK dlarfg convention) where the reference divides by zero.
+ */
+#ifndef TSQR_AMD_H
+#define TSQR_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define TSQR_OK 0
+#define TSQR_ERR_ARG (-1)       /* null pointer, m < n, n < 1, lda < m, ldr < n */
+#define TSQR_ERR_SHAPE (-2)     /* shape outside what the entry point supports */
+#define TSQR_ERR_WORKSPACE (-3) /* explicit workspace smaller than required */
+#define TSQR_ERR_HIP (-4)       /* a HIP runtime call or kernel launch failed */
+#define TSQR_ERR_ALLOC (-5)     /* internal workspace allocation failed */
+
+/* dtype codes follow the reference harness CLI (test_tsqr.cu:118-128). */
+#define TSQR_FLOAT 1
+#define TSQR_DOUBLE 2
+
+/* Human-readable text for a status code. */
+const char *tsqr_status_string(int status);
+
+/* Bytes of device workspace tsqr_* needs for an m x n problem of the given
+ * dtype (0 for an invalid shape).  Replaces the caller-side sizing
+ * ldwork = 2*NUM_SM*BLOCK_SIZE (test_tsqr.cu:42). */
+size_t tsqr_workspace_bytes(int m, int n, int dtype);
+
+/* Same for the recursive qr_* entry points. */
+size_t qr_workspace_bytes(int m, int n, int dtype);
+
+/* Tall-skinny Householder QR, replaces tsqr<float>/tsqr<double>
+ * (TallShinnyQR.h:86-149).  work holds ldwork*n elements. */
+int tsqr_s(int m, int n, float *A, int lda, float *R, int ldr, float *work, int ldwork);
+int tsqr_d(int m, int n, double *A, int lda, double *R, int ldr, double *work, int ldwork);
+
+/* Stream-ordered variants with an explicit byte-sized workspace; stream is a
+ * hipStream_t (NULL = null stream).  Return TSQR_ERR_WORKSPACE instead of
+ * falling back to the internal buffer. */
+int tsqr_s_stream(int m, int n, float *A, int lda, float *R, int ldr, void *work, size_t work_bytes,
+                  void *stream);
+int tsqr_d_stream(int m, int n, double *A, int lda, double *R, int ldr, void *work, size_t work_bytes,
+                  void *stream);
+
+/* Recursive blocked QR, replaces qr<float>/qr<double> (block_QR.h:72-95):
+ * n <= 32 is one tsqr; otherwise the columns are split in two, the halves
+ * are coupled by R12 = Q1^T A2 and A2 -= Q1 R12, and each half recurses. */
+int qr_s(int m, int n, float *A, int lda, float *R, int ldr, float *work, int ldwork);
+int qr_d(int m, int n, double *A, int lda, double *R, int ldr, double *work, int ldwork);
+int qr_s_stream(int m, int n, float *A, int lda, float *R, int ldr, void *work, size_t work_bytes, void *stream);
+int qr_d_stream(int m, int n, double *A, int lda, double *R, int ldr, void *work, size_t work_bytes,
+                void *stream);
+
+/* Per-phase timing (no counterpart in the reference, which only times whole
+ * calls with CUDA events, test_tsqr.cu:62-90).  While enabled, every call
+ * brackets each launch phase with HIP events on its stream:
+ *   phase 0 tsqr leaf R pass (or the whole single-tile call),
+ *   1 tsqr tree walk up, 2 tsqr down sweep (levels above the leaves),
+ *   3 tsqr leaf Q pass, 4 qr R12 = Q1^T A2, 5 qr A2 -= Q1 R12.
+ * tsqr_timing_read waits for the recorded events, writes the summed
+ * milliseconds and the number of intervals per phase (nphases <= 6 entries)
+ * and forgets them.  Enabling or disabling also forgets them.  Meant for
+ * benchmarks; it serialises nothing while recording. */
+int tsqr_timing_enable(int enable);
+int tsqr_timing_read(double *ms, int *count, int nphases);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSQR_AMD_H */

@@ -1,0 +1,60 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol
+include/tsqr_amd.h declares, and its pure host functions (workspace sizing,
+status text) behave.  No kernel is launched here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tsqr_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b([a-z_]+)\s*\(", text)) - {"if", "defined"})
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import tsqr_amd
+
+    return tsqr_amd.lib()
+
+
+def test_header_and_binding_agree():
+    from tsqr_amd._lib import EXPORTS
+
+    assert declared_functions() == sorted(EXPORTS)
+
+
+def test_every_declared_symbol_is_exported(lib):
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+        assert ctypes.cast(getattr(lib, name), ctypes.c_void_p).value
+
+
+def test_status_strings(lib):
+    assert lib.tsqr_status_string(0) == b"ok"
+    for code in (-1, -2, -3, -4, -5):
+        assert lib.tsqr_status_string(code) != b"unknown status"
+
+
+def test_workspace_sizing(lib):
+    # invalid shapes
+    assert lib.tsqr_workspace_bytes(10, 20, 1) == 0
+    assert lib.tsqr_workspace_bytes(100, 33, 1) == 0
+    assert lib.tsqr_workspace_bytes(100, 8, 3) == 0
+    # single tile: no tree, only the 256-byte minimum
+    assert lib.tsqr_workspace_bytes(256, 32, 1) == 256
+    # grows about linearly in m and is a small fraction of A (arity 8 for n=32 fp32)
+    a = lib.tsqr_workspace_bytes(1 << 20, 32, 1)
+    b = lib.tsqr_workspace_bytes(1 << 21, 32, 1)
+    assert 0 < a < b <= 2 * a + (1 << 20)
+    assert a < (1 << 20) * 32 * 4 // 6
+    # the maximum-height configuration fits in a fraction of A
+    big = lib.tsqr_workspace_bytes(23887872, 32, 1)
+    assert 0 < big < 23887872 * 32 * 4 // 6
+    assert lib.qr_workspace_bytes(27648, 128, 2) >= lib.tsqr_workspace_bytes(27648, 32, 2)

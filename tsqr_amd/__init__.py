@@ -1,0 +1,39 @@
+"""tsqr_amd -- MI355X (gfx950) tall-skinny and recursive blocked Householder QR.
+
+The compute lives in libtsqr_amd.so (hand-written HIP kernels behind the C
+ABI declared in include/tsqr_amd.h).  This module is a thin ctypes binding
+that takes torch tensors only as holders of device memory and streams:
+
+    tsqr(A, R=None, work=None)   A (m x n, column-major, on the GPU) is
+                                 overwritten by Q; returns (A, R)
+    qr(A, R=None, work=None)     recursive blocked QR, same convention
+
+A column-major m x n tensor is one whose strides are (1, lda), for example
+``colmajor(torch.randn(m, n))``.  Errors from the library raise TsqrError
+with the library's status code.  There is no CPU fallback: if the shared
+library is missing the import raises.
+"""
+from __future__ import annotations
+
+from ._lib import (  # noqa: F401
+    LIB_PATH,
+    TSQR_DOUBLE,
+    TSQR_FLOAT,
+    TsqrError,
+    lib,
+    qr_workspace_bytes,
+    tsqr_workspace_bytes,
+)
+from .api import colmajor, empty_colmajor, qr, tsqr  # noqa: F401
+
+__all__ = [
+    "LIB_PATH",
+    "TsqrError",
+    "colmajor",
+    "empty_colmajor",
+    "lib",
+    "qr",
+    "qr_workspace_bytes",
+    "tsqr",
+    "tsqr_workspace_bytes",
+]

@@ -1,0 +1,100 @@
+"""ctypes loader for libtsqr_amd.so (include/tsqr_amd.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtsqr_amd.so")
+
+TSQR_OK = 0
+TSQR_FLOAT = 1
+TSQR_DOUBLE = 2
+
+# Every symbol include/tsqr_amd.h declares.
+EXPORTS = (
+    "tsqr_status_string",
+    "tsqr_workspace_bytes",
+    "qr_workspace_bytes",
+    "tsqr_s",
+    "tsqr_d",
+    "tsqr_s_stream",
+    "tsqr_d_stream",
+    "qr_s",
+    "qr_d",
+    "qr_s_stream",
+    "qr_d_stream",
+    "tsqr_timing_enable",
+    "tsqr_timing_read",
+)
+
+
+class TsqrError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        super().__init__(f"{where}: status {status} ({status_string(status)})")
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m tsqr_amd.build` "
+            "(there is no CPU fallback)"
+        )
+    L = ctypes.CDLL(LIB_PATH)
+    i32, vp, sz = ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t
+    L.tsqr_status_string.argtypes = [i32]
+    L.tsqr_status_string.restype = ctypes.c_char_p
+    for f in (L.tsqr_workspace_bytes, L.qr_workspace_bytes):
+        f.argtypes = [i32, i32, i32]
+        f.restype = sz
+    for name in ("tsqr_s", "tsqr_d", "qr_s", "qr_d"):
+        f = getattr(L, name)
+        f.argtypes = [i32, i32, vp, i32, vp, i32, vp, i32]
+        f.restype = i32
+    for name in ("tsqr_s_stream", "tsqr_d_stream", "qr_s_stream", "qr_d_stream"):
+        f = getattr(L, name)
+        f.argtypes = [i32, i32, vp, i32, vp, i32, vp, sz, vp]
+        f.restype = i32
+    L.tsqr_timing_enable.argtypes = [i32]
+    L.tsqr_timing_enable.restype = i32
+    L.tsqr_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), i32]
+    L.tsqr_timing_read.restype = i32
+    _lib = L
+    return L
+
+
+PHASES = ("tsqr_leaf_r", "tsqr_tree_up", "tsqr_down", "tsqr_leafq", "qr_gemm_tn", "qr_gemm_nn")
+
+
+def timing_read() -> dict:
+    """Summed milliseconds and interval counts per phase since the last read."""
+    n = len(PHASES)
+    ms = (ctypes.c_double * n)()
+    cnt = (ctypes.c_int * n)()
+    st = lib().tsqr_timing_read(ms, cnt, n)
+    if st != TSQR_OK:
+        raise TsqrError(st, "tsqr_timing_read")
+    return {PHASES[i]: (ms[i], cnt[i]) for i in range(n)}
+
+
+def status_string(status: int) -> str:
+    try:
+        return lib().tsqr_status_string(status).decode()
+    except ImportError:
+        return "unknown"
+
+
+def tsqr_workspace_bytes(m: int, n: int, dtype_code: int) -> int:
+    return int(lib().tsqr_workspace_bytes(m, n, dtype_code))
+
+
+def qr_workspace_bytes(m: int, n: int, dtype_code: int) -> int:
+    return int(lib().qr_workspace_bytes(m, n, dtype_code))

@@ -1,0 +1,96 @@
+// plan.hpp -- host-side shape of the TSQR reduction tree and its workspace.
+//
+// The reference walks a tree whose level heights are H_{l+1} = ceil(H_l/BS)*n
+// (kernelQR.h:341-388) inside chunks of NUM_SM*BS rows, and adds a second
+// TSQR over the stacked chunk Rs when m is larger than one chunk
+// (TallShinnyQR.h:106-127).  Here the tree is one structure over all m rows:
+//   level 0   tiles of H = 64*RPL rows of A (the last one may be shorter);
+//   level l>0 blocks of arity*n rows, arity = floor(H/n); child b of level l
+//             contributes min(rows_b, n) rows of its R at row (b % arity)*n
+//             of block b / arity on level l+1;
+//   root      the first level with a single tile.
+// There is no chunking, hence no size cap: the tree just gets deeper.
+//
+// Workspace (all offsets 256-byte aligned):
+//   blocks   for every level l >= 1: tiles_l blocks of H x n elements,
+//            column-major with leading dimension H.  A block first holds the
+//            stacked child Rs, then its reflectors, then (after the down
+//            sweep) the S factors of its children.
+//   tickets  one int per tile of every level l >= 1 (zeroed per call).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+namespace tsqr {
+
+constexpr int kMaxLevels = 40;
+
+struct LevelDesc {
+    int64_t rows;     // valid rows on this level
+    int64_t blk_off;  // element offset of this level's blocks (levels >= 1)
+    int tiles;        // tiles on this level
+    int tile_rows;    // rows per full tile on this level
+    int ctr_off;      // offset of this level's tickets (levels >= 1)
+    int pad;
+};
+
+struct TreeDesc {
+    int m, n;
+    int H;       // tile height = 64 * RPL = leading dimension of a block
+    int arity;   // children per parent
+    int levels;  // number of levels (root is levels - 1)
+    int total_ctr;
+    int64_t block_elems;  // elements of all blocks
+    LevelDesc lv[kMaxLevels];
+};
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Builds the tree for an m x n problem with tile height H.  Returns false if
+// the shape cannot be represented (arity < 2 or too many levels).
+inline bool make_tree(int m, int n, int H, TreeDesc* d) {
+    d->m = m;
+    d->n = n;
+    d->H = H;
+    d->arity = H / n;
+    if (d->arity < 2) return false;
+    int l = 0;
+    int64_t rows = m;
+    int tile_rows = H;
+    int64_t blk = 0;
+    int ctr = 0;
+    for (;;) {
+        if (l >= kMaxLevels) return false;
+        LevelDesc& L = d->lv[l];
+        L.rows = rows;
+        L.tile_rows = tile_rows;
+        L.tiles = (int)((rows + tile_rows - 1) / tile_rows);
+        L.pad = 0;
+        if (l == 0) {
+            L.blk_off = 0;
+            L.ctr_off = 0;
+        } else {
+            L.blk_off = blk;
+            blk += (int64_t)L.tiles * H * n;
+            L.ctr_off = ctr;
+            ctr += L.tiles;
+        }
+        if (L.tiles == 1) break;
+        const int64_t last = rows - (int64_t)(L.tiles - 1) * tile_rows;
+        rows = (int64_t)(L.tiles - 1) * n + (last < n ? last : n);
+        tile_rows = d->arity * n;
+        ++l;
+    }
+    d->levels = l + 1;
+    d->total_ctr = ctr;
+    d->block_elems = blk;
+    return true;
+}
+
+// At least 256 bytes, so that 0 can mean "shape not supported".
+inline size_t tree_workspace_bytes(const TreeDesc& d, size_t elem) {
+    const size_t b = align256((size_t)d.block_elems * elem) + align256((size_t)d.total_ctr * sizeof(int));
+    return b < 256 ? 256 : b;
+}
+
+}  // namespace tsqr

@@ -1,0 +1,327 @@
+// tsqr_amd.hip -- host drivers and the C ABI of libtsqr_amd.so.
+//
+// tsqr_* : one TSQR of an m x n (n <= 32) column-major matrix; the kernels
+//          (tsqr_kernels.hpp) are compiled per dtype in tsqr_f32.hip and
+//          tsqr_f64.hip.
+// qr_*   : the reference's recursive column split (block_QR.h:72-95) with
+//          the coupling products of gemm_valu.hpp.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <mutex>
+#include <vector>
+
+#include "../../include/tsqr_amd.h"
+#include "gemm_valu.hpp"
+#include "tsqr_launch.hpp"
+
+namespace tsqr {
+
+// ------------------------------------------------------------------------
+// Phase timing: event pairs recorded on the launch stream, summed on read.
+// ------------------------------------------------------------------------
+namespace {
+std::atomic<bool> g_timing{false};
+std::mutex g_timing_mu;
+struct EventPair {
+    int phase;
+    hipEvent_t begin, end;
+};
+std::vector<EventPair> g_events;
+}  // namespace
+
+bool timing_enabled() { return g_timing.load(std::memory_order_relaxed); }
+
+void timing_mark(int phase, bool begin, hipStream_t st) {
+    std::lock_guard<std::mutex> lock(g_timing_mu);
+    if (begin) {
+        EventPair e{phase, nullptr, nullptr};
+        if (hipEventCreate(&e.begin) != hipSuccess || hipEventCreate(&e.end) != hipSuccess) return;
+        (void)hipEventRecord(e.begin, st);
+        g_events.push_back(e);
+    } else {
+        for (auto it = g_events.rbegin(); it != g_events.rend(); ++it) {
+            if (it->phase == phase) {
+                (void)hipEventRecord(it->end, st);
+                break;
+            }
+        }
+    }
+}
+
+namespace {
+
+struct Pool {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_pool_mu;
+Pool g_pool[64];
+
+// Per-device internal workspace for the reference-shaped entry points.
+int pool_get(size_t bytes, void** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return TSQR_ERR_HIP;
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    Pool& p = g_pool[dev];
+    if (p.bytes < bytes) {
+        if (p.ptr) {
+            if (hipDeviceSynchronize() != hipSuccess) return TSQR_ERR_HIP;
+            (void)hipFree(p.ptr);
+            p.ptr = nullptr;
+            p.bytes = 0;
+        }
+        if (hipMalloc(&p.ptr, bytes) != hipSuccess) {
+            p.ptr = nullptr;
+            return TSQR_ERR_ALLOC;
+        }
+        p.bytes = bytes;
+    }
+    *out = p.ptr;
+    return TSQR_OK;
+}
+
+template <typename T>
+int check_args(int m, int n, const T* A, int lda, const T* R, int ldr) {
+    if (!A || !R || n < 1 || m < n || lda < m || ldr < n) return TSQR_ERR_ARG;
+    return TSQR_OK;
+}
+
+template <typename T>
+size_t tsqr_ws_bytes(int m, int n) {
+    if (n < 1 || m < n || n > 32) return 0;
+    TreeDesc d;
+    if (!plan_tsqr<T>(m, n, &d)) return 0;
+    return tree_workspace_bytes(d, sizeof(T));
+}
+
+// Runs one TSQR; work must hold tsqr_ws_bytes<T>(m, n) bytes.
+template <typename T>
+int tsqr_core(int m, int n, T* A, int lda, T* R, int ldr, void* work, hipStream_t st) {
+    TreeDesc d;
+    if (!plan_tsqr<T>(m, n, &d)) return TSQR_ERR_SHAPE;
+    return tsqr_launch<T>(d, A, lda, R, ldr, work, st);
+}
+
+template <typename T>
+int tsqr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t work_bytes, hipStream_t st) {
+    int e = check_args(m, n, A, lda, R, ldr);
+    if (e) return e;
+    if (n > 32) return TSQR_ERR_SHAPE;
+    const size_t need = tsqr_ws_bytes<T>(m, n);
+    if (need == 0) return TSQR_ERR_SHAPE;
+    if (work_bytes < need || (need > 0 && !work)) return TSQR_ERR_WORKSPACE;
+    return tsqr_core<T>(m, n, A, lda, R, ldr, work, st);
+}
+
+// ------------------------------------------------------------------------
+// Recursive blocked QR.
+// ------------------------------------------------------------------------
+
+// Left block width of the split: n/2 for n = 32*2^k (block_QR.h:81-83),
+// a multiple of 32 in general.
+inline int split_cols(int n) { return 32 * ((n + 63) / 64); }
+
+inline int tn_chunk_rows(int m) {
+    int kc = (m + 1023) / 1024;
+    kc = (kc + 31) / 32 * 32;
+    return kc < 1024 ? 1024 : kc;
+}
+
+template <typename T>
+size_t qr_ws_bytes(int m, int n) {
+    if (n < 1 || m < n) return 0;
+    if (n <= 32) return tsqr_ws_bytes<T>(m, n);
+    const size_t panel = tsqr_ws_bytes<T>(m, 32);
+    const int n1 = split_cols(n), n2 = n - n1;
+    const int kc = tn_chunk_rows(m);
+    const size_t chunks = (size_t)((m + kc - 1) / kc);
+    return align256(panel) + align256(chunks * (size_t)n1 * n2 * sizeof(T));
+}
+
+template <typename T>
+int qr_rec(int m, int n, T* A, int lda, T* R, int ldr, char* ws, size_t panel_bytes, T* part, hipStream_t st) {
+    if (n <= 32) return tsqr_core<T>(m, n, A, lda, R, ldr, ws, st);
+    const int n1 = split_cols(n), n2 = n - n1;
+    T* A1 = A;
+    T* A2 = A + (int64_t)n1 * lda;
+    T* R11 = R;
+    T* R12 = R + (int64_t)n1 * ldr;
+    T* R22 = R + (int64_t)n1 * ldr + n1;
+    int e = qr_rec<T>(m, n1, A1, lda, R11, ldr, ws, panel_bytes, part, st);
+    if (e) return e;
+    const int kc = tn_chunk_rows(m);
+    const int chunks = (m + kc - 1) / kc;
+    const bool timed = timing_enabled();
+    if (timed) timing_mark(kPhaseGemmTN, true, st);
+    hipLaunchKernelGGL(gemm_tn_partial<T>, dim3(chunks, (n1 + 63) / 64, (n2 + 63) / 64), dim3(256), 0, st, m, n1, n2,
+                       A1, (int64_t)lda, A2, (int64_t)lda, kc, part);
+    hipLaunchKernelGGL(gemm_tn_reduce<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, n1, n2, chunks, part, R12,
+                       (int64_t)ldr);
+    if (timed) timing_mark(kPhaseGemmTN, false, st);
+    if (timed) timing_mark(kPhaseGemmNN, true, st);
+    hipLaunchKernelGGL(gemm_nn_update<T>, dim3((m + 255) / 256, (n2 + 63) / 64), dim3(256), 0, st, m, n1, n2, A1,
+                       (int64_t)lda, R12, (int64_t)ldr, A2, (int64_t)lda);
+    if (timed) timing_mark(kPhaseGemmNN, false, st);
+    hipLaunchKernelGGL(zero_block<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, R, (int64_t)ldr, n1, n2, 0, n1);
+    if (hipGetLastError() != hipSuccess) return TSQR_ERR_HIP;
+    return qr_rec<T>(m, n2, A2, lda, R22, ldr, ws, panel_bytes, part, st);
+}
+
+template <typename T>
+int qr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t work_bytes, hipStream_t st) {
+    int e = check_args(m, n, A, lda, R, ldr);
+    if (e) return e;
+    const size_t need = qr_ws_bytes<T>(m, n);
+    if (need == 0) return TSQR_ERR_SHAPE;
+    if (work_bytes < need || !work) return TSQR_ERR_WORKSPACE;
+    if (n <= 32) return tsqr_core<T>(m, n, A, lda, R, ldr, work, st);
+    const size_t panel = align256(tsqr_ws_bytes<T>(m, 32));
+    char* ws = reinterpret_cast<char*>(work);
+    return qr_rec<T>(m, n, A, lda, R, ldr, ws, panel, reinterpret_cast<T*>(ws + panel), st);
+}
+
+template <typename T>
+int with_ref_workspace(int m, int n, T* work, int ldwork, size_t need, void** ws, size_t* bytes) {
+    const size_t have = (work && ldwork > 0) ? (size_t)ldwork * (size_t)n * sizeof(T) : 0;
+    if (have >= need && ((uintptr_t)work % 256) == 0) {
+        *ws = work;
+        *bytes = have;
+        return TSQR_OK;
+    }
+    *bytes = need;
+    return pool_get(need, ws);
+}
+
+}  // namespace
+}  // namespace tsqr
+
+using namespace tsqr;
+
+extern "C" {
+
+const char* tsqr_status_string(int status) {
+    switch (status) {
+        case TSQR_OK:
+            return "ok";
+        case TSQR_ERR_ARG:
+            return "invalid argument (null pointer, m < n, n < 1, lda < m or ldr < n)";
+        case TSQR_ERR_SHAPE:
+            return "shape not supported by this entry point";
+        case TSQR_ERR_WORKSPACE:
+            return "workspace smaller than required";
+        case TSQR_ERR_HIP:
+            return "HIP runtime error";
+        case TSQR_ERR_ALLOC:
+            return "internal workspace allocation failed";
+        default:
+            return "unknown status";
+    }
+}
+
+int tsqr_timing_enable(int enable) {
+    std::lock_guard<std::mutex> lock(g_timing_mu);
+    for (auto& e : g_events) {
+        (void)hipEventDestroy(e.begin);
+        (void)hipEventDestroy(e.end);
+    }
+    g_events.clear();
+    g_timing.store(enable != 0);
+    return TSQR_OK;
+}
+
+int tsqr_timing_read(double* ms, int* count, int nphases) {
+    if (!ms || !count || nphases < 1) return TSQR_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_timing_mu);
+    for (int p = 0; p < nphases; ++p) {
+        ms[p] = 0.0;
+        count[p] = 0;
+    }
+    int status = TSQR_OK;
+    for (auto& e : g_events) {
+        float t = 0.f;
+        if (hipEventSynchronize(e.end) != hipSuccess || hipEventElapsedTime(&t, e.begin, e.end) != hipSuccess)
+            status = TSQR_ERR_HIP;
+        else if (e.phase < nphases) {
+            ms[e.phase] += t;
+            count[e.phase] += 1;
+        }
+        (void)hipEventDestroy(e.begin);
+        (void)hipEventDestroy(e.end);
+    }
+    g_events.clear();
+    return status;
+}
+
+size_t tsqr_workspace_bytes(int m, int n, int dtype) {
+    if (dtype == TSQR_FLOAT) return tsqr_ws_bytes<float>(m, n);
+    if (dtype == TSQR_DOUBLE) return tsqr_ws_bytes<double>(m, n);
+    return 0;
+}
+
+size_t qr_workspace_bytes(int m, int n, int dtype) {
+    if (dtype == TSQR_FLOAT) return qr_ws_bytes<float>(m, n);
+    if (dtype == TSQR_DOUBLE) return qr_ws_bytes<double>(m, n);
+    return 0;
+}
+
+int tsqr_s_stream(int m, int n, float* A, int lda, float* R, int ldr, void* work, size_t work_bytes, void* stream) {
+    return tsqr_stream<float>(m, n, A, lda, R, ldr, work, work_bytes, (hipStream_t)stream);
+}
+int tsqr_d_stream(int m, int n, double* A, int lda, double* R, int ldr, void* work, size_t work_bytes,
+                  void* stream) {
+    return tsqr_stream<double>(m, n, A, lda, R, ldr, work, work_bytes, (hipStream_t)stream);
+}
+int qr_s_stream(int m, int n, float* A, int lda, float* R, int ldr, void* work, size_t work_bytes, void* stream) {
+    return qr_stream<float>(m, n, A, lda, R, ldr, work, work_bytes, (hipStream_t)stream);
+}
+int qr_d_stream(int m, int n, double* A, int lda, double* R, int ldr, void* work, size_t work_bytes, void* stream) {
+    return qr_stream<double>(m, n, A, lda, R, ldr, work, work_bytes, (hipStream_t)stream);
+}
+
+int tsqr_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int ldwork) {
+    int e = check_args(m, n, A, lda, R, ldr);
+    if (e) return e;
+    const size_t need = tsqr_ws_bytes<float>(m, n);
+    if (need == 0) return TSQR_ERR_SHAPE;
+    void* ws = nullptr;
+    size_t bytes = 0;
+    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
+    if (e) return e;
+    return tsqr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+}
+int tsqr_d(int m, int n, double* A, int lda, double* R, int ldr, double* work, int ldwork) {
+    int e = check_args(m, n, A, lda, R, ldr);
+    if (e) return e;
+    const size_t need = tsqr_ws_bytes<double>(m, n);
+    if (need == 0) return TSQR_ERR_SHAPE;
+    void* ws = nullptr;
+    size_t bytes = 0;
+    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
+    if (e) return e;
+    return tsqr_stream<double>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+}
+int qr_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int ldwork) {
+    int e = check_args(m, n, A, lda, R, ldr);
+    if (e) return e;
+    const size_t need = qr_ws_bytes<float>(m, n);
+    if (need == 0) return TSQR_ERR_SHAPE;
+    void* ws = nullptr;
+    size_t bytes = 0;
+    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
+    if (e) return e;
+    return qr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+}
+int qr_d(int m, int n, double* A, int lda, double* R, int ldr, double* work, int ldwork) {
+    int e = check_args(m, n, A, lda, R, ldr);
+    if (e) return e;
+    const size_t need = qr_ws_bytes<double>(m, n);
+    if (need == 0) return TSQR_ERR_SHAPE;
+    void* ws = nullptr;
+    size_t bytes = 0;
+    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
+    if (e) return e;
+    return qr_stream<double>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+}
+
+}  // extern "C"

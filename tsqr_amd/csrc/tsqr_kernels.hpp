@@ -1,0 +1,278 @@
+// tsqr_kernels.hpp -- the TSQR up sweep, down sweep and leaf-Q kernels.
+//
+// Every workgroup is one wavefront that owns one tile of the tree (plan.hpp).
+//
+// tsqr_leaf_r  level-0 tiles of A are factored (R only; A is not written)
+//              and each R goes into its slot of the parent block.
+// tsqr_tree_up levels >= 1 in one launch: each wave factors a level-1
+//              block, and the last child to finish a parent (atomic ticket)
+//              factors the parent, so the rest of the reduction tree is
+//              walked inside this one launch with no grid barrier and no
+//              waiting (the reference spins on a global counter between
+//              levels, kernelQR.h:367-375).  Blocks keep their reflectors in
+//              place.  The root writes R to the caller and its own explicit
+//              Q, which holds the n x n S factors of its children.
+// tsqr_single  m <= one tile: factor, R, and Q in one wave.
+// tsqr_down   levels root-1 .. 1: Q_tile (from the stored reflectors) times
+//             the S factor handed down by the parent gives the S factors of
+//             the tile's children, written over the tile's block.
+// tsqr_leafq  level 0: the tile of A is factored again (same code, same
+//             input, bit-identical reflectors), Q_tile is formed and
+//             multiplied by its S factor, and the result is the final Q for
+//             those rows of A.  This is the only write of A.
+//
+// Traffic per call: A is read twice and written once, plus the level >= 1
+// blocks (about m*n/(arity-1) elements) written and read a few times.  The
+// reference instead round-trips every level's Q through a work array and,
+// for m above one chunk, makes a third full pass over A (tsgemm_tsqr,
+// TallShinnyQR.h:11-74, :126-127).
+#pragma once
+#include "plan.hpp"
+#include "tile.hpp"
+
+namespace tsqr {
+
+template <typename T, int RPL>
+struct Chunk16 {
+    static constexpr int kElems = 16 / (int)sizeof(T);
+    static constexpr int kChunks = RPL / kElems;
+    static_assert(RPL % kElems == 0, "a lane's rows must be a whole number of 16-byte chunks");
+    typedef T vec __attribute__((ext_vector_type(16 / sizeof(T))));
+};
+
+// x[s][c] = base[(row0 + s) + c*ld] for rows < `rows` and columns < ncols,
+// zero elsewhere.  VEC: base and ld*sizeof(T) are 16-byte aligned.
+template <typename T, int N, int RPL, bool VEC>
+__device__ __forceinline__ void load_rows(T (&x)[RPL][N], const T* __restrict__ base, int64_t ld,
+                                          int rows, int ncols, int lane) {
+    using C = Chunk16<T, RPL>;
+    const int row0 = lane * RPL;
+    const bool full = row0 + RPL <= rows;
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+        if (c < ncols) {
+            const T* p = base + (int64_t)c * ld + row0;
+            if (VEC && full) {
+#pragma unroll
+                for (int q = 0; q < C::kChunks; ++q) {
+                    const typename C::vec v = *reinterpret_cast<const typename C::vec*>(p + q * C::kElems);
+#pragma unroll
+                    for (int e = 0; e < C::kElems; ++e) x[q * C::kElems + e][c] = v[e];
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) x[s][c] = (row0 + s < rows) ? p[s] : T(0);
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < RPL; ++s) x[s][c] = T(0);
+        }
+    }
+}
+
+template <typename T, int RPL, bool VEC>
+__device__ __forceinline__ void store_column(const T (&v)[RPL], T* __restrict__ p, int rows, int row0) {
+    using C = Chunk16<T, RPL>;
+    if (VEC && row0 + RPL <= rows) {
+#pragma unroll
+        for (int q = 0; q < C::kChunks; ++q) {
+            typename C::vec w;
+#pragma unroll
+            for (int e = 0; e < C::kElems; ++e) w[e] = v[q * C::kElems + e];
+            *reinterpret_cast<typename C::vec*>(p + q * C::kElems) = w;
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < RPL; ++s)
+            if (row0 + s < rows) p[s] = v[s];
+    }
+}
+
+template <typename T, int N, int RPL, bool VEC>
+__device__ __forceinline__ void store_rows(const T (&x)[RPL][N], T* __restrict__ base, int64_t ld, int rows,
+                                           int ncols, int lane) {
+    const int row0 = lane * RPL;
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+        if (c < ncols) {
+            T v[RPL];
+#pragma unroll
+            for (int s = 0; s < RPL; ++s) v[s] = x[s][c];
+            store_column<T, RPL, VEC>(v, base + (int64_t)c * ld + row0, rows, row0);
+        }
+    }
+}
+
+// dst[:, c] = Q_tile[:, 0:kmax] * S[0:kmax, c] for c < n.
+template <typename T, int N, int RPL, bool VEC>
+__device__ __forceinline__ void apply_s_store(const T (&x)[RPL][N], const T* __restrict__ S, int ldS, int kmax,
+                                              int n, T* __restrict__ dst, int64_t ldd, int rows, int lane) {
+    const int row0 = lane * RPL;
+#pragma unroll 1
+    for (int c = 0; c < n; ++c) {
+        T acc[RPL];
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) acc[s] = T(0);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            if (k < kmax) {
+                const T sv = S[k + (int64_t)c * ldS];
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) acc[s] = fmat(x[s][k], sv, acc[s]);
+            }
+        }
+        store_column<T, RPL, VEC>(acc, dst + (int64_t)c * ldd + row0, rows, row0);
+    }
+}
+
+// R (n x n, zero below the diagonal) from the wave's LDS to dst (ld),
+// rows [0, nrows).
+template <typename T, int N>
+__device__ __forceinline__ void store_r(const TileLds<T, N>& L, T* __restrict__ dst, int64_t ld, int nrows, int n,
+                                        int lane) {
+    if (lane < nrows) {
+#pragma unroll 1
+        for (int c = 0; c < n; ++c) dst[lane + (int64_t)c * ld] = (lane <= c) ? L.R[lane + c * N] : T(0);
+    }
+}
+
+__device__ __forceinline__ int tile_rows_of(const LevelDesc& L, int tile) {
+    const int64_t left = L.rows - (int64_t)tile * L.tile_rows;
+    return (int)(left < L.tile_rows ? left : L.tile_rows);
+}
+
+// Level 0 (at least two levels): R of each tile of A into its slot of the
+// parent block.  A is only read.
+template <typename T, int N, int RPL, bool VEC>
+__global__ __launch_bounds__(64) void tsqr_leaf_r(const TreeDesc d, const T* __restrict__ A, int lda,
+                                                   T* __restrict__ blocks) {
+    __shared__ TileLds<T, N> L;
+    const int lane = threadIdx.x;
+    const int n = d.n;
+    const int tile = blockIdx.x;
+    const int rows = tile_rows_of(d.lv[0], tile);
+    T x[RPL][N];
+    load_rows<T, N, RPL, VEC>(x, A + (int64_t)tile * d.H, lda, rows, n, lane);
+    factor_tile<T, N, RPL>(x, L, lane);
+    const LevelDesc& P = d.lv[1];
+    const int parent = tile / d.arity;
+    const int slot = tile - parent * d.arity;
+    store_r<T, N>(L, blocks + P.blk_off + (int64_t)parent * d.H * n + slot * n, d.H, rows < n ? rows : n, n, lane);
+}
+
+// Levels >= 1, one launch: a wave factors one level-1 block, keeps its
+// reflectors in place and hands its R to the parent block; the last child
+// of a parent to finish (ticket) goes on to factor the parent, and so on up
+// to the root.  Nothing waits: a wave that is not last simply exits.  The
+// root writes R to the caller and its explicit Q (the S factors of its
+// children) over its block.
+template <typename T, int N, int RPL>
+__global__ __launch_bounds__(64) void tsqr_tree_up(const TreeDesc d, T* __restrict__ R, int ldr,
+                                                    T* __restrict__ blocks, int* __restrict__ tickets) {
+    __shared__ TileLds<T, N> L;
+    const int lane = threadIdx.x;
+    const int n = d.n;
+    int level = 1;
+    int tile = blockIdx.x;
+    int rows = tile_rows_of(d.lv[1], tile);
+    T* blk = blocks + d.lv[1].blk_off + (int64_t)tile * d.H * n;
+    T x[RPL][N];
+    load_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
+#pragma unroll 1
+    for (;;) {
+        factor_tile<T, N, RPL>(x, L, lane);
+        const LevelDesc& Lv = d.lv[level];
+        if (Lv.tiles == 1) break;
+        store_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
+        const LevelDesc& P = d.lv[level + 1];
+        const int parent = tile / d.arity;
+        const int slot = tile - parent * d.arity;
+        T* pblk = blocks + P.blk_off + (int64_t)parent * d.H * n;
+        store_r<T, N>(L, pblk + slot * n, d.H, rows < n ? rows : n, n, lane);
+
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        int old = 0;
+        if (lane == 0)
+            old = __hip_atomic_fetch_add(&tickets[P.ctr_off + parent], 1, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        old = __builtin_amdgcn_readfirstlane(old);
+        const int left = Lv.tiles - parent * d.arity;
+        const int nchild = left < d.arity ? left : d.arity;
+        if (old != nchild - 1) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+        ++level;
+        tile = parent;
+        blk = pblk;
+        rows = tile_rows_of(P, parent);
+        load_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
+    }
+    store_r<T, N>(L, R, ldr, n, n, lane);
+    form_q_tile<T, N, RPL>(x, L, lane);
+    store_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
+}
+
+// A single tile (m <= H): factor, R to the caller, Q over A.
+template <typename T, int N, int RPL, bool VEC>
+__global__ __launch_bounds__(64) void tsqr_single(const TreeDesc d, T* __restrict__ A, int lda, T* __restrict__ R,
+                                                   int ldr) {
+    __shared__ TileLds<T, N> L;
+    const int lane = threadIdx.x;
+    const int n = d.n;
+    const int rows = (int)d.lv[0].rows;
+    T x[RPL][N];
+    load_rows<T, N, RPL, VEC>(x, A, lda, rows, n, lane);
+    factor_tile<T, N, RPL>(x, L, lane);
+    store_r<T, N>(L, R, ldr, n, n, lane);
+    form_q_tile<T, N, RPL>(x, L, lane);
+    store_rows<T, N, RPL, VEC>(x, A, lda, rows, n, lane);
+}
+
+template <typename T, int N, int RPL>
+__global__ __launch_bounds__(64) void tsqr_down(const TreeDesc d, int level, T* __restrict__ blocks) {
+    __shared__ TileLds<T, N> L;
+    const int lane = threadIdx.x;
+    const int n = d.n;
+    const int tile = blockIdx.x;
+    const LevelDesc& Lv = d.lv[level];
+    const LevelDesc& P = d.lv[level + 1];
+    const int rows = tile_rows_of(Lv, tile);
+    T* myblk = blocks + Lv.blk_off + (int64_t)tile * d.H * n;
+    T x[RPL][N];
+    load_rows<T, N, RPL, true>(x, myblk, d.H, rows, n, lane);
+    form_q_tile<T, N, RPL>(x, L, lane);
+    const int parent = tile / d.arity;
+    const int slot = tile - parent * d.arity;
+    const T* S = blocks + P.blk_off + (int64_t)parent * d.H * n + slot * n;
+    apply_s_store<T, N, RPL, true>(x, S, d.H, rows < n ? rows : n, n, myblk, d.H, rows, lane);
+}
+
+template <typename T, int N, int RPL, bool VEC>
+__global__ __launch_bounds__(64) void tsqr_leafq(const TreeDesc d, T* __restrict__ A, int lda,
+                                                  const T* __restrict__ blocks) {
+    __shared__ TileLds<T, N> L;
+    const int lane = threadIdx.x;
+    const int n = d.n;
+    const int tile = blockIdx.x;
+    const int rows = tile_rows_of(d.lv[0], tile);
+    T* a = A + (int64_t)tile * d.H;
+    T x[RPL][N];
+    load_rows<T, N, RPL, VEC>(x, a, lda, rows, n, lane);
+    factor_tile<T, N, RPL>(x, L, lane);
+    form_q_tile<T, N, RPL>(x, L, lane);
+    const LevelDesc& P = d.lv[1];
+    const int parent = tile / d.arity;
+    const int slot = tile - parent * d.arity;
+    const T* S = blocks + P.blk_off + (int64_t)parent * d.H * n + slot * n;
+    apply_s_store<T, N, RPL, VEC>(x, S, d.H, rows < n ? rows : n, n, a, lda, rows, lane);
+}
+
+// One copy per translation unit (each dtype's TU launches its own).
+namespace {
+__global__ void zero_ints(int* __restrict__ p, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
+}
+}  // namespace
+
+}  // namespace tsqr
