@@ -13,9 +13,10 @@ re-uploaded between steps.
 metric  TSQR HBM GB/s = B_alg / (seconds per step), with
         B_alg = (3mn + n^2) * 4 bytes (A read twice and written once are the
         lower bound for an explicit Q that does not fit on chip).
-roofline  for the dominant kernel (tsqr_leafq: reads A and writes Q,
-        2mn*4 algorithmic bytes per launch) from HIP events on the launch
-        stream over the timed steps, against the 8 TB/s HBM peak.
+roofline  for the dominant phase of the two streaming passes (leaf R pass:
+        reads A, m*n*4 algorithmic bytes; leaf Q pass: reads A and writes Q,
+        2*m*n*4), from HIP events on the launch stream over the timed steps,
+        against the 8 TB/s HBM peak.
 cpu_baseline  the oracle's C restatement of the reference algorithm
         (OpenMP over tiles, all host cores) on a bounded sample of the same
         workload (m/8 rows), plus OpenBLAS sgeqrf+sorgqr on the same sample.
@@ -55,6 +56,8 @@ def parse():
     p.add_argument("--workload", default="tsqr_max_fp32", choices=sorted(WORKLOADS))
     p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     p.add_argument("--cpu-fraction", type=int, default=8, help="cpu sample = m / this")
+    p.add_argument("--leaf-path", choices=("default", "auto", "householder"), default="default",
+                   help="level-0 superleaf path (default: the library's default for the dtype)")
     return p.parse_args()
 
 
@@ -115,7 +118,8 @@ def main():
     import torch
 
     import tsqr_amd
-    from tsqr_amd._lib import TSQR_DOUBLE, TSQR_FLOAT, timing_read
+    from tsqr_amd._lib import (TSQR_DOUBLE, TSQR_FLOAT, TSQR_LEAF_AUTO, TSQR_LEAF_HOUSEHOLDER, leaf_stats,
+                               set_leaf_path, timing_read)
 
     kind, m, n, dtype = WORKLOADS[args.workload]
     tdt = torch.float32 if dtype == "float32" else torch.float64
@@ -134,6 +138,8 @@ def main():
     need = (L.tsqr_workspace_bytes if kind == "tsqr" else L.qr_workspace_bytes)(m, n, code)
     work = torch.empty(need, dtype=torch.uint8, device=dev)
     fn = getattr(L, f"{kind}_{'s' if code == TSQR_FLOAT else 'd'}_stream")
+    if args.leaf_path != "default":
+        set_leaf_path(code, TSQR_LEAF_AUTO if args.leaf_path == "auto" else TSQR_LEAF_HOUSEHOLDER)
     stream = torch.cuda.current_stream(dev)
 
     def step():
@@ -167,7 +173,7 @@ def main():
     # Algorithmic bytes per launch of the two streaming kernels: the leaf R
     # pass reads A (m*n elements), the leaf Q pass reads A and writes Q.
     panel_n = n if kind == "tsqr" else 32
-    alg = {"tsqr_leaf_r": m * panel_n * esz, "tsqr_leafq": 2 * m * panel_n * esz}
+    alg = {"leaf_r": m * panel_n * esz, "leaf_q": 2 * m * panel_n * esz}
     kern = {}
     for name, nbytes in alg.items():
         tot_ms, cnt = phases[name]
@@ -214,6 +220,8 @@ def main():
             "algorithmic_bytes_per_step": balg,
             "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
             "phase_ms": phase_ms,
+            "superleaves": dict(zip(("gram", "householder"), leaf_stats(work.data_ptr(), m, n, code)))
+            if kind == "tsqr" else None,
         },
         "roofline": roofline,
     }

@@ -32,10 +32,36 @@ def from_device_colmajor(buf, m: int, n: int, lda: int, offset_elems: int = 0) -
     return v.cpu().numpy().T.copy()
 
 
+class LeafPath:
+    """Context manager selecting the level-0 leaf path for one dtype."""
+
+    def __init__(self, dtype, path: int, gate: float = 1024.0):
+        from tsqr_amd._lib import TSQR_DOUBLE, TSQR_FLOAT
+
+        self.code = TSQR_FLOAT if np.dtype(dtype) == np.float32 else TSQR_DOUBLE
+        self.path, self.gate = path, gate
+
+    def __enter__(self):
+        from tsqr_amd._lib import TSQR_LEAF_AUTO, TSQR_LEAF_HOUSEHOLDER, TSQR_FLOAT, set_leaf_path
+
+        set_leaf_path(self.code, self.path, self.gate)
+        self.default = (TSQR_LEAF_AUTO if self.code == TSQR_FLOAT else TSQR_LEAF_HOUSEHOLDER, 1024.0)
+        return self
+
+    def __exit__(self, *exc):
+        from tsqr_amd._lib import set_leaf_path
+
+        set_leaf_path(self.code, *self.default)
+
+
+LAST_LEAF_STATS = {}
+
+
 def call(kind: str, A: np.ndarray, lda: int | None = None, offset_elems: int = 0, work_bytes: int | None = None,
          ref_shaped: bool = False, ldwork: int = 0):
     """Runs tsqr/qr on a copy of A through the C ABI.  Returns (status, Q, R,
-    padding_untouched)."""
+    padding_untouched); for tsqr through the stream entry point the Gram /
+    Householder superleaf counts land in LAST_LEAF_STATS."""
     import torch
 
     import tsqr_amd
@@ -56,6 +82,10 @@ def call(kind: str, A: np.ndarray, lda: int | None = None, offset_elems: int = 0
         nbytes = need if work_bytes is None else work_bytes
         work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device="cuda")
         st = getattr(L, f"{kind}_{suf}_stream")(m, n, pA, lda, Rbuf.data_ptr(), n, work.data_ptr(), nbytes, None)
+        if st == 0 and kind == "tsqr":
+            from tsqr_amd._lib import leaf_stats
+
+            LAST_LEAF_STATS["gram"], LAST_LEAF_STATS["householder"] = leaf_stats(work.data_ptr(), m, n, code)
     torch.cuda.synchronize()
     Q = from_device_colmajor(buf, m, n, lda, offset_elems)
     R = Rbuf.view(n, n).cpu().numpy().T.copy()

@@ -4,6 +4,9 @@ Bar (per result, stated in each assertion): with u the unit roundoff of the
 working precision and n the column count,
   backward error ||A-QR||_F/||A||_F        <= 8 n u
   orthogonality  ||I-Q^TQ||_F              <= 8 n u
+(n is taken as at least 32 for FP64 on the Gram leaf path, whose Gram matrix
+has no wider type to accumulate in: its errors grow like sqrt(leaf rows) u
+rather than n u)
   R vs FP64 LAPACK after sign normalisation <= 8 n u kappa_est (random inputs
   are well conditioned, kappa_est = cond_2(A) from the FP64 SVD)
 and the reference's own all_close(QR, A) (abs 1e-4, rel 1e-5, utils.h:352-357).
@@ -12,14 +15,17 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.gpu_util import call
+from tests.gpu_util import LAST_LEAF_STATS, LeafPath, call
 
 pytestmark = pytest.mark.gpu
 
+AUTO, HOUSEHOLDER = 0, 1
+PATHS = [pytest.param(AUTO, id="auto"), pytest.param(HOUSEHOLDER, id="householder")]
 
-def check(A, Q, R, tag, c=8.0):
+
+def check(A, Q, R, tag, c=8.0, n_floor=1):
     u = O.unit_roundoff(A.dtype)
-    n = A.shape[1]
+    n = max(A.shape[1], n_floor)
     Qo, Ro = O.lapack_qr(A)
     kappa = np.linalg.cond(A.astype(np.float64))
     m = O.metrics(A, Q, R, Ro, Qo)
@@ -35,13 +41,16 @@ def check(A, Q, R, tag, c=8.0):
     return m
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_cfg1_harness_input(dtype):
+def test_cfg1_harness_input(dtype, path):
     """m=27648, n=32 with the reference harness's input stream."""
     A = O.gen_uniform(27648, 32, dtype)
-    st, Q, R, _ = call("tsqr", A)
+    with LeafPath(dtype, path):
+        st, Q, R, _ = call("tsqr", A)
     assert st == 0
-    check(A, Q, R, f"cfg1 {np.dtype(dtype).name}")
+    check(A, Q, R, f"cfg1 {np.dtype(dtype).name} path={path}",
+          n_floor=32 if (dtype == np.float64 and path == AUTO) else 1)
     # the reference's restatement at its A800 tuning agrees after sign normalisation
     Qr, Rr = O.tsqr_ref(A, bs=256, num_sm=108)
     _, Rn = O.sign_normalise(None, R)
@@ -64,13 +73,16 @@ SHAPES = [
 ]
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("m,n", SHAPES)
-def test_shapes(m, n, dtype):
+def test_shapes(m, n, dtype, path):
     A = O.gen_normal(m, n, dtype, seed=m * 7 + n)
-    st, Q, R, _ = call("tsqr", A)
+    with LeafPath(dtype, path):
+        st, Q, R, _ = call("tsqr", A)
     assert st == 0
-    check(A, Q, R, f"{m}x{n} {np.dtype(dtype).name}")
+    floor = 32 if (dtype == np.float64 and path == AUTO) else 1
+    check(A, Q, R, f"{m}x{n} {np.dtype(dtype).name} path={path} leaves={LAST_LEAF_STATS}", n_floor=floor)
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
@@ -131,3 +143,39 @@ def test_reference_shaped_entry_small_ldwork(dtype):
     st, Q, R, _ = call("tsqr", A, ref_shaped=True, ldwork=2 * 108 * 128)
     assert st == 0
     check(A, Q, R, "ref-shaped")
+
+
+def test_gram_path_is_taken_for_well_conditioned_fp32():
+    A = O.gen_normal(200000, 32, np.float32, seed=21)
+    with LeafPath(np.float32, AUTO):
+        st, Q, R, _ = call("tsqr", A)
+    assert st == 0
+    assert LAST_LEAF_STATS["householder"] == 0 and LAST_LEAF_STATS["gram"] > 1
+    check(A, Q, R, f"gram path leaves={LAST_LEAF_STATS}")
+
+
+@pytest.mark.parametrize("cond", [1e4, 1e6])
+def test_ill_conditioned_falls_back_to_householder(cond):
+    """cond(A) far above the gate: every superleaf is rejected by the Gram
+    path, and the Householder result meets the kappa-independent bounds."""
+    A = O.gen_conditioned(60000, 32, cond, np.float32, seed=5)
+    with LeafPath(np.float32, AUTO):
+        st, Q, R, _ = call("tsqr", A)
+    assert st == 0
+    assert LAST_LEAF_STATS["gram"] == 0
+    check(A, Q, R, f"cond={cond:g} leaves={LAST_LEAF_STATS}")
+
+
+def test_mixed_leaf_paths_in_one_call():
+    """One superleaf made nearly rank deficient (two columns almost equal on
+    its rows only) goes Householder, the others stay on the Gram path."""
+    m, n = 40000, 32
+    A = O.gen_normal(m, n, np.float32, seed=8)
+    leaf = 2048  # FP32, n = 32: 8 tiles of 256 rows
+    rows = slice(3 * leaf, 4 * leaf)
+    A[rows, 7] = A[rows, 6] * (1 + 1e-6)
+    with LeafPath(np.float32, AUTO):
+        st, Q, R, _ = call("tsqr", A)
+    assert st == 0
+    assert LAST_LEAF_STATS["householder"] == 1 and LAST_LEAF_STATS["gram"] == (m + leaf - 1) // leaf - 1
+    check(A, Q, R, f"mixed leaves={LAST_LEAF_STATS}")

@@ -10,6 +10,8 @@ LIB_PATH = os.path.join(HERE, "libtsqr_amd.so")
 TSQR_OK = 0
 TSQR_FLOAT = 1
 TSQR_DOUBLE = 2
+TSQR_LEAF_AUTO = 0
+TSQR_LEAF_HOUSEHOLDER = 1
 
 # Every symbol include/tsqr_amd.h declares.
 EXPORTS = (
@@ -24,6 +26,8 @@ EXPORTS = (
     "qr_d",
     "qr_s_stream",
     "qr_d_stream",
+    "tsqr_set_leaf_path",
+    "tsqr_leaf_stats",
     "tsqr_timing_enable",
     "tsqr_timing_read",
 )
@@ -63,6 +67,10 @@ def lib() -> ctypes.CDLL:
         f = getattr(L, name)
         f.argtypes = [i32, i32, vp, i32, vp, i32, vp, sz, vp]
         f.restype = i32
+    L.tsqr_set_leaf_path.argtypes = [i32, i32, ctypes.c_double]
+    L.tsqr_set_leaf_path.restype = i32
+    L.tsqr_leaf_stats.argtypes = [vp, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.tsqr_leaf_stats.restype = i32
     L.tsqr_timing_enable.argtypes = [i32]
     L.tsqr_timing_enable.restype = i32
     L.tsqr_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), i32]
@@ -71,7 +79,7 @@ def lib() -> ctypes.CDLL:
     return L
 
 
-PHASES = ("tsqr_leaf_r", "tsqr_tree_up", "tsqr_down", "tsqr_leafq", "qr_gemm_tn", "qr_gemm_nn")
+PHASES = ("leaf_r", "tree_up", "down", "leaf_q", "qr_gemm_tn", "qr_gemm_nn")
 
 
 def timing_read() -> dict:
@@ -83,6 +91,21 @@ def timing_read() -> dict:
     if st != TSQR_OK:
         raise TsqrError(st, "tsqr_timing_read")
     return {PHASES[i]: (ms[i], cnt[i]) for i in range(n)}
+
+
+def set_leaf_path(dtype_code: int, path: int, gate: float = 1024.0) -> None:
+    st = lib().tsqr_set_leaf_path(dtype_code, path, gate)
+    if st != TSQR_OK:
+        raise TsqrError(st, "tsqr_set_leaf_path")
+
+
+def leaf_stats(work_ptr: int, m: int, n: int, dtype_code: int) -> tuple:
+    """(Gram superleaves, Householder superleaves) of the last call on work."""
+    g, h = ctypes.c_int(0), ctypes.c_int(0)
+    st = lib().tsqr_leaf_stats(work_ptr, m, n, dtype_code, ctypes.byref(g), ctypes.byref(h))
+    if st != TSQR_OK:
+        raise TsqrError(st, "tsqr_leaf_stats")
+    return g.value, h.value
 
 
 def status_string(status: int) -> str:

@@ -1,0 +1,430 @@
+// gram.hpp -- the Gram (CholeskyQR) path for well-conditioned superleaves.
+//
+// A superleaf is a block of super_tiles*H consecutive rows of A (H = tile
+// height of the Householder kernels, super_tiles = min(8, H/N)).  For such a block A_b the R factor is unique up to
+// the signs of its rows, so it can be obtained either by Householder
+// reflections (super.hpp) or from the Gram matrix:
+//   G_b = A_b^T A_b   accumulated in FP64 on the matrix cores
+//                     (v_mfma_f64_16x16x4f64; products of FP32 inputs are
+//                     exact in FP64),
+//   R_b = chol(G_b)   FP64, one wavefront, positive diagonal.
+// With FP64 accumulation the relative error of R_b is O(u64 * kappa(A_b)^2),
+// below FP32 roundoff while kappa(A_b) <= ~1e4.  The tree receives R_b
+// rounded to the working precision (R~_b); the Q side is
+//   Q_b = A_b * (R_b^{-1} S_b)
+// with S_b the n x n block of the tree's Q for this leaf, accumulated in FP64
+// and rounded once.  A_b R_b^{-1} is orthonormal to O(u64 kappa^2), and the
+// residual A_b - (A_b R_b^{-1}) R~_b = (A_b R_b^{-1})(R_b - R~_b) is at the
+// working roundoff, so neither error grows with kappa in that range.  Each
+// superleaf computes
+//   kappa_est = ||R_b||_F * ||R_b^{-1}||_F  (an upper bound of kappa_2(A_b))
+// and takes the Householder path instead (super.hpp) when the Cholesky
+// breaks down or kappa_est exceeds the gate.
+//
+// MFMA operand layout (16x16x4 f64): lane l feeds A[i = l%16][k = l/16] and
+// B[k = l/16][j = l%16]; the accumulator register v of lane l holds
+// C[4v + l/16][l%16].  A lane loads 4 consecutive rows of one column with one
+// 16-byte load; MFMA number t (t = 0..3) takes row 4*(l/16)+t as its k index,
+// and the four MFMAs together cover 16 rows.
+#pragma once
+#include "tsqr_kernels.hpp"
+
+namespace tsqr {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// LDS of one Gram wave: the 32 x 32 Gram / R matrix (row-major, padded) and
+// one broadcast row.
+struct GramLds {
+    double G[32][33];
+    double row[32];
+};
+
+// Four consecutive rows of one column in the storage type.
+template <typename T>
+struct Raw4 {
+    T v[4];
+};
+
+// Rows [r0, r0+4) of column c of the (rows x ncols) block at base, zero
+// outside; the caller converts to double when the value is consumed.
+template <typename T, bool VEC>
+__device__ __forceinline__ Raw4<T> load_raw4(const T* __restrict__ base, int64_t ld, int r0, int c, int rows,
+                                             int ncols) {
+    Raw4<T> o;
+    if (c < ncols) {
+        const T* p = base + (int64_t)c * ld + r0;
+        if (VEC && r0 + 4 <= rows) {
+            if constexpr (sizeof(T) == 4) {
+                typedef float f4 __attribute__((ext_vector_type(4)));
+                const f4 q = *reinterpret_cast<const f4*>(p);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) o.v[t] = q[t];
+            } else {
+                typedef double d2 __attribute__((ext_vector_type(2)));
+                const d2 q0 = *reinterpret_cast<const d2*>(p);
+                const d2 q1 = *reinterpret_cast<const d2*>(p + 2);
+                o.v[0] = q0[0];
+                o.v[1] = q0[1];
+                o.v[2] = q1[0];
+                o.v[3] = q1[1];
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) o.v[t] = (r0 + t < rows) ? p[t] : T(0);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) o.v[t] = T(0);
+    }
+    return o;
+}
+
+// Loads rows [r0, r0+4) of column c of the (rows x ncols) block at base into
+// v (as double), zero outside.
+template <typename T, bool VEC>
+__device__ __forceinline__ void load4(double (&v)[4], const T* __restrict__ base, int64_t ld, int r0, int c,
+                                      int rows, int ncols) {
+    if (c < ncols) {
+        const T* p = base + (int64_t)c * ld + r0;
+        if (VEC && r0 + 4 <= rows) {
+            if constexpr (sizeof(T) == 4) {
+                typedef float f4 __attribute__((ext_vector_type(4)));
+                const f4 q = *reinterpret_cast<const f4*>(p);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) v[t] = (double)q[t];
+            } else {
+                typedef double d2 __attribute__((ext_vector_type(2)));
+                const d2 q0 = *reinterpret_cast<const d2*>(p);
+                const d2 q1 = *reinterpret_cast<const d2*>(p + 2);
+                v[0] = q0[0];
+                v[1] = q0[1];
+                v[2] = q1[0];
+                v[3] = q1[1];
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[t] = (r0 + t < rows) ? (double)p[t] : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) v[t] = 0.0;
+    }
+}
+
+// Cholesky of L.G (n x n, n <= 32) in place of its upper triangle:
+// L.G[j][c] = R[j][c] for c >= j, R^T R = G.  Columns >= n are treated as an
+// identity block.  Returns false if a pivot is not positive and finite.
+// Lane c < 32 owns column c in g[].
+__device__ __forceinline__ bool chol_wave(GramLds& L, int n, int lane, double (&g)[32]) {
+    const int c = lane & 31;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) g[r] = (r < n && c < n) ? L.G[r][c] : (r == c ? 1.0 : 0.0);
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+        double piv = readlane_t<double>(g[j], j);
+        if (!(piv > 0.0) || !(piv < __builtin_inf())) {
+            ok = false;
+            piv = 1.0;
+        }
+        const double rd = 1.0 / __builtin_sqrt(piv);
+        // row j of R: R[j][c] = G[j][c] / sqrt(G[j][j]) for c >= j
+        g[j] = (c >= j) ? g[j] * rd : 0.0;
+        if (lane < 32) L.row[c] = g[j];
+        wave_sync();
+        // G[r][c] -= R[j][r] R[j][c] for j < r <= c
+#pragma unroll
+        for (int r = j + 1; r < 32; ++r) g[r] = (r <= c) ? fmat(-L.row[r], g[j], g[r]) : g[r];
+        wave_sync();
+    }
+    return ok;
+}
+
+// Copies R (lane c < 32 holds column c in g[]) to L.G, row-major.
+__device__ __forceinline__ void r_to_lds(const double (&g)[32], GramLds& L, int lane) {
+    if (lane < 32) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) L.G[r][lane] = g[r];
+    }
+    wave_sync();
+}
+
+// Inverse of the upper triangular R in L.G (row-major): lane c back-
+// substitutes R x = e_c, so x[] is column c of R^{-1}.
+__device__ __forceinline__ void tri_inv_wave(const GramLds& L, int lane, double (&x)[32]) {
+    const int c = lane & 31;
+#pragma unroll
+    for (int i = 31; i >= 0; --i) {
+        double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = i + 1; k < 32; ++k) s = fmat(-L.G[i][k], x[k], s);
+        x[i] = (i <= c) ? s / L.G[i][i] : 0.0;
+    }
+}
+
+__device__ __forceinline__ int leaf_rows_of(const LevelDesc& L, int leaf) {
+    const int64_t left = L.rows - (int64_t)leaf * L.tile_rows;
+    return (int)(left < L.tile_rows ? left : L.tile_rows);
+}
+
+// Sum over lanes 0..31 (lanes 32..63 hold copies and contribute zero).
+__device__ __forceinline__ double sum32(double v, int lane) {
+    double p[1] = {lane < 32 ? v : 0.0};
+    return wave_sum_transposed<1>(p, lane);
+}
+
+// Pass 1a of the Gram path, one wavefront per superleaf: G = A_b^T A_b in
+// FP64 (MFMA), written to side[leaf] (32 x 32, row-major).  A streaming
+// kernel: the only state is the 12 accumulator registers and the slabs in
+// flight.
+template <typename T, int NT, bool VEC>
+__global__ __launch_bounds__(64) void tsqr_gram_g(const TreeDesc d, const T* __restrict__ A, int lda,
+                                                   double* __restrict__ side) {
+    const int lane = threadIdx.x;
+    const int n = d.n;
+    const int leaf = blockIdx.x;
+    const LevelDesc& L0 = d.lv[0];
+    const int rows = leaf_rows_of(L0, leaf);
+    const T* base = A + (int64_t)leaf * L0.tile_rows;
+    v4d acc00 = {0, 0, 0, 0}, acc01 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+    const int col = lane & 15;
+    const int rq = 4 * (lane >> 4);
+    constexpr int D = 4;  // slabs of 16 rows per batch of loads
+    int s0 = 0;
+    if (VEC) {
+        // full batches: D*16 rows, no predication
+#pragma unroll 1
+        for (; s0 + D * 16 <= rows; s0 += D * 16) {
+            Raw4<T> a0[D], a1[D];
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+                a0[q] = load_raw4<T, true>(base, lda, s0 + 16 * q + rq, col, rows, n);
+                if constexpr (NT == 2) a1[q] = load_raw4<T, true>(base, lda, s0 + 16 * q + rq, col + 16, rows, n);
+            }
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const double x0 = (double)a0[q].v[t];
+                    acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, x0, acc00, 0, 0, 0);
+                    if constexpr (NT == 2) {
+                        const double x1 = (double)a1[q].v[t];
+                        acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, x1, acc01, 0, 0, 0);
+                        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, x1, acc11, 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll 1
+    for (; s0 < rows; s0 += 16) {
+        double a0[4], a1[4];
+        load4<T, VEC>(a0, base, lda, s0 + rq, col, rows, n);
+        if constexpr (NT == 2) load4<T, VEC>(a1, base, lda, s0 + rq, col + 16, rows, n);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], a0[t], acc00, 0, 0, 0);
+            if constexpr (NT == 2) {
+                acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], a1[t], acc01, 0, 0, 0);
+                acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], a1[t], acc11, 0, 0, 0);
+            }
+        }
+    }
+    // G row-major 32 x 32; lane l, register v holds row 4v + l/16, column l%16
+    double* G = side + (int64_t)leaf * 1024;
+    const int i0 = lane >> 4;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int i = 4 * v + i0;
+        G[i * 32 + col] = acc00[v];
+        G[i * 32 + 16 + col] = NT == 2 ? acc01[v] : 0.0;
+        G[(16 + col) * 32 + i] = NT == 2 ? acc01[v] : 0.0;
+        G[(16 + i) * 32 + 16 + col] = NT == 2 ? acc11[v] : 0.0;
+    }
+}
+
+// Pass 1b, one wavefront per superleaf: R = chol(G), X = R^{-1},
+// kappa_est = ||R||_F ||R^{-1}||_F.
+// Success: R rounded to T goes to the superleaf's slot in its parent block
+// (or to the caller's R if the superleaf is the root) and X (FP64, 32 x 32,
+// row-major) replaces G in side[leaf]; flags[leaf] stays 0.
+// Failure (pivot not positive, or kappa_est > gate): flags[leaf] = 1 and
+// nothing else is written; the Householder superleaf kernels take over.
+template <typename T>
+__global__ __launch_bounds__(64) void tsqr_gram_chol(const TreeDesc d, T* __restrict__ R, int ldr,
+                                                      T* __restrict__ blocks, double* __restrict__ side,
+                                                      int* __restrict__ flags, double gate) {
+    __shared__ GramLds L;
+    const int lane = threadIdx.x;
+    const int n = d.n;
+    const int leaf = blockIdx.x;
+    const LevelDesc& L0 = d.lv[0];
+    double* G = side + (int64_t)leaf * 1024;
+    {
+        const int c = lane & 31;
+        if (lane < 32) {
+#pragma unroll 8
+            for (int r = 0; r < 32; ++r) L.G[r][c] = G[r * 32 + c];
+        }
+        wave_sync();
+    }
+    double nr;
+    bool ok;
+    {
+        double g[32];
+        ok = chol_wave(L, n, lane, g);
+        nr = 0.0;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) nr = fmat(g[r], g[r], nr);
+        r_to_lds(g, L, lane);
+    }
+    double x[32];
+    tri_inv_wave(L, lane, x);
+    const int c = lane & 31;
+    double nx = 0.0;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) nx = fmat(x[r], x[r], nx);
+    if (c >= n) {
+        nr = 0.0;
+        nx = 0.0;
+    }
+    const double kest = __builtin_sqrt(sum32(nr, lane) * sum32(nx, lane));
+    ok = ok && (kest <= gate);  // false for NaN too
+    if (!ok) {
+        if (lane == 0) flags[leaf] = 1;
+        return;
+    }
+    T* dst;
+    int64_t ld;
+    if (L0.tiles == 1) {
+        dst = R;
+        ld = ldr;
+    } else {
+        const int parent = leaf / d.arity;
+        const int slot = leaf - parent * d.arity;
+        dst = blocks + d.lv[1].blk_off + (int64_t)parent * d.H * n + slot * n;
+        ld = d.H;
+    }
+    if (lane < n) {
+#pragma unroll 4
+        for (int r = 0; r < n; ++r) dst[r + (int64_t)lane * ld] = (r <= lane) ? (T)L.G[r][lane] : T(0);
+    }
+    // X row-major: lane c holds column c
+    if (lane < 32) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) G[r * 32 + c] = x[r];
+    }
+}
+
+// Pass 2 of the Gram path, one wavefront per superleaf.
+// Prologue: M = X S on the matrix cores (X = R^{-1} from side[leaf], S the
+// superleaf's block of the tree's Q, identity at the root).  The MFMA
+// accumulator layout of M (lane l, register v of tile (i0, j0) holds
+// M[i0 + 4v + l/16][j0 + l%16]) is exactly the B-operand layout of the
+// k-group k0 = i0 + 4v, so M stays in registers as mb[k-group][j-tile].
+// Rows: blocks of 64 rows; lane l loads rows 4(l%16)..+3 of column
+// k0 + l/16 with one 16-byte load, MFMA number t takes row 4(l%16)+t, and
+// the accumulator register v of (t, j-tile) is row 16v + 4(l/16) + t,
+// column j0 + l%16 of the output, so each lane stores 4 consecutive rows of
+// one column per (v, j-tile).  Products and sums in FP64, one rounding.
+template <typename T, bool VEC>
+__global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restrict__ A, int lda,
+                                                   const double* __restrict__ side, const T* __restrict__ blocks,
+                                                   const int* __restrict__ flags) {
+    const int leaf = blockIdx.x;
+    if (flags[leaf]) return;
+    const int lane = threadIdx.x;
+    const int n = d.n;
+    const LevelDesc& L0 = d.lv[0];
+    const int li = lane & 15, lq = lane >> 4;
+    // ---- M = X S ----
+    const double* X = side + (int64_t)leaf * 1024;  // row-major
+    const bool root = L0.tiles == 1;
+    const T* S = nullptr;
+    if (!root) {
+        const int parent = leaf / d.arity;
+        const int slot = leaf - parent * d.arity;
+        S = blocks + d.lv[1].blk_off + (int64_t)parent * d.H * n + slot * n;  // ld H
+    }
+    double mb[8][2];
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) {
+            v4d acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int kg = 0; kg < 8; ++kg) {
+                const int k = 4 * kg + lq;
+                const double xa = X[(16 * it + li) * 32 + k];
+                const int jc = 16 * jt + li;
+                double sb;
+                if (root)
+                    sb = (k == jc && k < n) ? 1.0 : 0.0;
+                else
+                    sb = (k < n && jc < n) ? (double)S[k + (int64_t)jc * d.H] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, sb, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) mb[4 * it + v][jt] = acc[v];
+        }
+    }
+    // ---- Q rows = A rows * M ----
+    const int rows = leaf_rows_of(L0, leaf);
+    T* base = A + (int64_t)leaf * L0.tile_rows;
+    const int nk = (n + 3) / 4;  // k-groups holding real columns
+    // raw[kg]: this lane's 4 input rows of column 4 kg + l/16 for the block
+    // being computed; the next block's loads are issued before the MFMAs.
+    Raw4<T> raw[8];
+#pragma unroll
+    for (int kg = 0; kg < 8; ++kg) {
+        const int k = 4 * kg + lq;
+        if (kg < nk) raw[kg] = load_raw4<T, VEC>(base, lda, 4 * li, k < n ? k : n - 1, rows, n);
+    }
+#pragma unroll 1
+    for (int r0 = 0; r0 < rows; r0 += 64) {
+        double a[8][4];
+#pragma unroll
+        for (int kg = 0; kg < 8; ++kg)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) a[kg][t] = (double)raw[kg].v[t];
+        const int r1 = r0 + 64;
+        if (r1 < rows) {
+#pragma unroll
+            for (int kg = 0; kg < 8; ++kg) {
+                const int k = 4 * kg + lq;
+                if (kg < nk) raw[kg] = load_raw4<T, VEC>(base, lda, r1 + 4 * li, k < n ? k : n - 1, rows, n);
+            }
+        }
+        v4d acc[4][2];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = v4d{0, 0, 0, 0};
+#pragma unroll
+        for (int kg = 0; kg < 8; ++kg) {
+            if (kg < nk) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg][t], mb[kg][0], acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg][t], mb[kg][1], acc[t][1], 0, 0, 0);
+                }
+            }
+        }
+        // acc[t][jt][v] = Q[r0 + 16v + 4 lq + t][16 jt + li]
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) {
+            const int cc = 16 * jt + li;
+            if (cc < n) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    T q[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) q[t] = (T)acc[t][jt][v];
+                    store_column<T, 4, VEC>(q, base + (int64_t)cc * lda + r0 + 16 * v + 4 * lq, rows,
+                                            r0 + 16 * v + 4 * lq);
+                }
+            }
+        }
+    }
+}
+
+}  // namespace tsqr

@@ -4,19 +4,17 @@
 // (kernelQR.h:341-388) inside chunks of NUM_SM*BS rows, and adds a second
 // TSQR over the stacked chunk Rs when m is larger than one chunk
 // (TallShinnyQR.h:106-127).  Here the tree is one structure over all m rows:
-//   level 0   tiles of H = 64*RPL rows of A (the last one may be shorter);
+//   level 0   superleaves of leaf_rows = 8*H rows of A (the last one may be
+//             shorter), H = 64*RPL the Householder tile height;
 //   level l>0 blocks of arity*n rows, arity = floor(H/n); child b of level l
 //             contributes min(rows_b, n) rows of its R at row (b % arity)*n
 //             of block b / arity on level l+1;
 //   root      the first level with a single tile.
 // There is no chunking, hence no size cap: the tree just gets deeper.
 //
-// Workspace (all offsets 256-byte aligned):
-//   blocks   for every level l >= 1: tiles_l blocks of H x n elements,
-//            column-major with leading dimension H.  A block first holds the
-//            stacked child Rs, then its reflectors, then (after the down
-//            sweep) the S factors of its children.
-//   tickets  one int per tile of every level l >= 1 (zeroed per call).
+// Blocks of level l >= 1: tiles_l blocks of H x n elements, column-major
+// with leading dimension H.  A block first holds the stacked child Rs, then
+// its reflectors, then (after the down sweep) the S factors of its children.
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
@@ -46,9 +44,10 @@ struct TreeDesc {
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Builds the tree for an m x n problem with tile height H.  Returns false if
-// the shape cannot be represented (arity < 2 or too many levels).
-inline bool make_tree(int m, int n, int H, TreeDesc* d) {
+// Builds the tree for an m x n problem: level-0 tiles of leaf_rows rows,
+// blocks of H rows above.  Returns false if the shape cannot be represented
+// (arity < 2 or too many levels).
+inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d) {
     d->m = m;
     d->n = n;
     d->H = H;
@@ -56,7 +55,7 @@ inline bool make_tree(int m, int n, int H, TreeDesc* d) {
     if (d->arity < 2) return false;
     int l = 0;
     int64_t rows = m;
-    int tile_rows = H;
+    int tile_rows = leaf_rows;
     int64_t blk = 0;
     int ctr = 0;
     for (;;) {
@@ -87,10 +86,26 @@ inline bool make_tree(int m, int n, int H, TreeDesc* d) {
     return true;
 }
 
-// At least 256 bytes, so that 0 can mean "shape not supported".
-inline size_t tree_workspace_bytes(const TreeDesc& d, size_t elem) {
-    const size_t b = align256((size_t)d.block_elems * elem) + align256((size_t)d.total_ctr * sizeof(int));
-    return b < 256 ? 256 : b;
+// Workspace layout of one TSQR call (byte offsets, 256-byte aligned):
+//   blocks   tree blocks of levels >= 1 (elements of T)
+//   tickets  one int per tile of levels >= 1
+//   flags    one int per level-0 superleaf: 0 Gram path, 1 Householder path
+//   side     32 x 32 FP64 per superleaf: R^{-1}, then R^{-1} S (Gram path)
+struct WsLayout {
+    size_t blocks, tickets, flags, side, total;
+};
+
+inline WsLayout ws_layout(const TreeDesc& d, size_t elem) {
+    WsLayout w;
+    const size_t leaves = (size_t)d.lv[0].tiles;
+    w.blocks = 0;
+    w.tickets = align256((size_t)d.block_elems * elem);
+    w.flags = w.tickets + align256((size_t)d.total_ctr * sizeof(int));
+    w.side = w.flags + align256(leaves * sizeof(int));
+    w.total = w.side + align256(leaves * 1024 * sizeof(double));
+    return w;
 }
+
+inline size_t tree_workspace_bytes(const TreeDesc& d, size_t elem) { return ws_layout(d, elem).total; }
 
 }  // namespace tsqr

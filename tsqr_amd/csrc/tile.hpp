@@ -42,20 +42,6 @@ struct TileLds {
     T R[N * N];  // upper triangle of R, column-major, leading dimension N
 };
 
-template <typename T>
-__device__ __forceinline__ T readlane_t(T v, int lane);
-template <>
-__device__ __forceinline__ float readlane_t<float>(float v, int lane) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
-}
-template <>
-__device__ __forceinline__ double readlane_t<double>(double v, int lane) {
-    uint64_t u = __builtin_bit_cast(uint64_t, v);
-    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
-    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
-    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
 // ---------------------------------------------------------------------------
 // Factorization: reflector J applied to the trailing columns J+1..N-1.
 // ---------------------------------------------------------------------------

@@ -95,12 +95,25 @@ size_t tsqr_ws_bytes(int m, int n) {
     return tree_workspace_bytes(d, sizeof(T));
 }
 
+// Leaf path per dtype (tsqr_set_leaf_path).  FP32 defaults to the Gram path
+// with the Householder fallback; FP64 has no wider type to accumulate the
+// Gram matrix in, so it defaults to Householder leaves.
+std::mutex g_opts_mu;
+LaunchOpts g_opts_f32{kLeafAuto, 1024.0};
+LaunchOpts g_opts_f64{kLeafHouseholder, 1024.0};
+
+template <typename T>
+LaunchOpts current_opts() {
+    std::lock_guard<std::mutex> lock(g_opts_mu);
+    return sizeof(T) == 4 ? g_opts_f32 : g_opts_f64;
+}
+
 // Runs one TSQR; work must hold tsqr_ws_bytes<T>(m, n) bytes.
 template <typename T>
 int tsqr_core(int m, int n, T* A, int lda, T* R, int ldr, void* work, hipStream_t st) {
     TreeDesc d;
     if (!plan_tsqr<T>(m, n, &d)) return TSQR_ERR_SHAPE;
-    return tsqr_launch<T>(d, A, lda, R, ldr, work, st);
+    return tsqr_launch<T>(d, A, lda, R, ldr, work, current_opts<T>(), st);
 }
 
 template <typename T>
@@ -251,6 +264,42 @@ int tsqr_timing_read(double* ms, int* count, int nphases) {
     }
     g_events.clear();
     return status;
+}
+
+int tsqr_set_leaf_path(int dtype, int path, double gate) {
+    if ((path != TSQR_LEAF_AUTO && path != TSQR_LEAF_HOUSEHOLDER) || !(gate >= 1.0)) return TSQR_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_opts_mu);
+    if (dtype == TSQR_FLOAT)
+        g_opts_f32 = LaunchOpts{path, gate};
+    else if (dtype == TSQR_DOUBLE)
+        g_opts_f64 = LaunchOpts{path, gate};
+    else
+        return TSQR_ERR_ARG;
+    return TSQR_OK;
+}
+
+int tsqr_leaf_stats(const void* work, int m, int n, int dtype, int* gram_leaves, int* householder_leaves) {
+    if (!work || !gram_leaves || !householder_leaves || (dtype != TSQR_FLOAT && dtype != TSQR_DOUBLE))
+        return TSQR_ERR_ARG;
+    if (n < 1 || m < n || n > 32) return TSQR_ERR_SHAPE;
+    TreeDesc d;
+    const size_t elem = dtype == TSQR_FLOAT ? sizeof(float) : sizeof(double);
+    const bool ok = dtype == TSQR_FLOAT ? plan_tsqr<float>(m, n, &d) : plan_tsqr<double>(m, n, &d);
+    if (!ok) return TSQR_ERR_SHAPE;
+    *gram_leaves = 0;
+    *householder_leaves = 0;
+    if (d.lv[0].rows <= d.H) {  // single tile: always Householder
+        *householder_leaves = 1;
+        return TSQR_OK;
+    }
+    const WsLayout w = ws_layout(d, elem);
+    std::vector<int> flags(d.lv[0].tiles);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(flags.data(), reinterpret_cast<const char*>(work) + w.flags, flags.size() * sizeof(int),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+        return TSQR_ERR_HIP;
+    for (int f : flags) (f ? *householder_leaves : *gram_leaves) += 1;
+    return TSQR_OK;
 }
 
 size_t tsqr_workspace_bytes(int m, int n, int dtype) {

@@ -2,5 +2,5 @@
 #include "tsqr_launch_impl.hpp"
 
 namespace tsqr {
-template int tsqr_launch<float>(const TreeDesc&, float*, int, float*, int, void*, hipStream_t);
+template int tsqr_launch<float>(const TreeDesc&, float*, int, float*, int, void*, const LaunchOpts&, hipStream_t);
 }

@@ -2,5 +2,5 @@
 #include "tsqr_launch_impl.hpp"
 
 namespace tsqr {
-template int tsqr_launch<double>(const TreeDesc&, double*, int, double*, int, void*, hipStream_t);
+template int tsqr_launch<double>(const TreeDesc&, double*, int, double*, int, void*, const LaunchOpts&, hipStream_t);
 }

@@ -2,8 +2,6 @@
 //
 // Every workgroup is one wavefront that owns one tile of the tree (plan.hpp).
 //
-// tsqr_leaf_r  level-0 tiles of A are factored (R only; A is not written)
-//              and each R goes into its slot of the parent block.
 // tsqr_tree_up levels >= 1 in one launch: each wave factors a level-1
 //              block, and the last child to finish a parent (atomic ticket)
 //              factors the parent, so the rest of the reduction tree is
@@ -16,13 +14,10 @@
 // tsqr_down   levels root-1 .. 1: Q_tile (from the stored reflectors) times
 //             the S factor handed down by the parent gives the S factors of
 //             the tile's children, written over the tile's block.
-// tsqr_leafq  level 0: the tile of A is factored again (same code, same
-//             input, bit-identical reflectors), Q_tile is formed and
-//             multiplied by its S factor, and the result is the final Q for
-//             those rows of A.  This is the only write of A.
 //
-// Traffic per call: A is read twice and written once, plus the level >= 1
-// blocks (about m*n/(arity-1) elements) written and read a few times.  The
+// Level 0 (superleaves of A) is handled by gram.hpp and super.hpp.  Traffic
+// per call: A is read twice and written once, plus the level >= 1 blocks
+// (about m*n/(8*(arity-1)) elements) written and read a few times.  The
 // reference instead round-trips every level's Q through a work array and,
 // for m above one chunk, makes a third full pass over A (tsgemm_tsqr,
 // TallShinnyQR.h:11-74, :126-127).
@@ -141,25 +136,6 @@ __device__ __forceinline__ int tile_rows_of(const LevelDesc& L, int tile) {
     return (int)(left < L.tile_rows ? left : L.tile_rows);
 }
 
-// Level 0 (at least two levels): R of each tile of A into its slot of the
-// parent block.  A is only read.
-template <typename T, int N, int RPL, bool VEC>
-__global__ __launch_bounds__(64) void tsqr_leaf_r(const TreeDesc d, const T* __restrict__ A, int lda,
-                                                   T* __restrict__ blocks) {
-    __shared__ TileLds<T, N> L;
-    const int lane = threadIdx.x;
-    const int n = d.n;
-    const int tile = blockIdx.x;
-    const int rows = tile_rows_of(d.lv[0], tile);
-    T x[RPL][N];
-    load_rows<T, N, RPL, VEC>(x, A + (int64_t)tile * d.H, lda, rows, n, lane);
-    factor_tile<T, N, RPL>(x, L, lane);
-    const LevelDesc& P = d.lv[1];
-    const int parent = tile / d.arity;
-    const int slot = tile - parent * d.arity;
-    store_r<T, N>(L, blocks + P.blk_off + (int64_t)parent * d.H * n + slot * n, d.H, rows < n ? rows : n, n, lane);
-}
-
 // Levels >= 1, one launch: a wave factors one level-1 block, keeps its
 // reflectors in place and hands its R to the parent block; the last child
 // of a parent to finish (ticket) goes on to factor the parent, and so on up
@@ -245,26 +221,6 @@ __global__ __launch_bounds__(64) void tsqr_down(const TreeDesc d, int level, T* 
     const int slot = tile - parent * d.arity;
     const T* S = blocks + P.blk_off + (int64_t)parent * d.H * n + slot * n;
     apply_s_store<T, N, RPL, true>(x, S, d.H, rows < n ? rows : n, n, myblk, d.H, rows, lane);
-}
-
-template <typename T, int N, int RPL, bool VEC>
-__global__ __launch_bounds__(64) void tsqr_leafq(const TreeDesc d, T* __restrict__ A, int lda,
-                                                  const T* __restrict__ blocks) {
-    __shared__ TileLds<T, N> L;
-    const int lane = threadIdx.x;
-    const int n = d.n;
-    const int tile = blockIdx.x;
-    const int rows = tile_rows_of(d.lv[0], tile);
-    T* a = A + (int64_t)tile * d.H;
-    T x[RPL][N];
-    load_rows<T, N, RPL, VEC>(x, a, lda, rows, n, lane);
-    factor_tile<T, N, RPL>(x, L, lane);
-    form_q_tile<T, N, RPL>(x, L, lane);
-    const LevelDesc& P = d.lv[1];
-    const int parent = tile / d.arity;
-    const int slot = tile - parent * d.arity;
-    const T* S = blocks + P.blk_off + (int64_t)parent * d.H * n + slot * n;
-    apply_s_store<T, N, RPL, VEC>(x, S, d.H, rows < n ? rows : n, n, a, lda, rows, lane);
 }
 
 // One copy per translation unit (each dtype's TU launches its own).

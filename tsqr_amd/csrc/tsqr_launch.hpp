@@ -18,18 +18,38 @@ constexpr int rows_per_lane(int N) {
     return 512 / (N * (int)sizeof(T));
 }
 
+// Householder tiles per level-0 superleaf: up to 8, and no more than fit
+// their stacked N x N R factors into one H-row tile (H/N of them).
+template <typename T>
+constexpr int super_tiles(int N) {
+    return (64 * rows_per_lane<T>(N)) / N < 8 ? (64 * rows_per_lane<T>(N)) / N : 8;
+}
+
 template <typename T>
 inline bool plan_tsqr(int m, int n, TreeDesc* d) {
-    return make_tree(m, n, 64 * rows_per_lane<T>(padded_cols(n)), d);
+    const int N = padded_cols(n);
+    const int H = 64 * rows_per_lane<T>(N);
+    return make_tree(m, n, H, super_tiles<T>(N) * H, d);
 }
+
+// How the level-0 superleaves are factored.
+enum LeafPath {
+    kLeafAuto = 0,         // Gram path, Householder where the gate rejects it
+    kLeafHouseholder = 1,  // Householder for every superleaf
+};
+
+struct LaunchOpts {
+    int path;     // LeafPath
+    double gate;  // largest accepted ||R||_F ||R^{-1}||_F on the Gram path
+};
 
 // Optional per-phase timing (tsqr_timing_* in the C ABI).  When enabled,
 // the launchers bracket each phase with HIP events on the launch stream.
 enum Phase {
-    kPhaseUp = 0,
-    kPhaseTree = 1,
-    kPhaseDown = 2,
-    kPhaseLeafQ = 3,
+    kPhaseLeafR = 0,  // Gram R pass + Householder superleaf R pass
+    kPhaseTree = 1,   // tree walk up
+    kPhaseDown = 2,   // tree down sweep + Gram M products
+    kPhaseLeafQ = 3,  // Gram Q pass + Householder superleaf Q pass
     kPhaseGemmTN = 4,
     kPhaseGemmNN = 5,
     kNumPhases = 6
@@ -40,9 +60,11 @@ void timing_mark(int phase, bool begin, hipStream_t st);
 // Enqueues one TSQR (plan d) on stream st; work holds
 // tree_workspace_bytes(d, sizeof(T)) bytes.  Returns a TSQR_* status.
 template <typename T>
-int tsqr_launch(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, hipStream_t st);
+int tsqr_launch(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, const LaunchOpts& o, hipStream_t st);
 
-extern template int tsqr_launch<float>(const TreeDesc&, float*, int, float*, int, void*, hipStream_t);
-extern template int tsqr_launch<double>(const TreeDesc&, double*, int, double*, int, void*, hipStream_t);
+extern template int tsqr_launch<float>(const TreeDesc&, float*, int, float*, int, void*, const LaunchOpts&,
+                                       hipStream_t);
+extern template int tsqr_launch<double>(const TreeDesc&, double*, int, double*, int, void*, const LaunchOpts&,
+                                        hipStream_t);
 
 }  // namespace tsqr

@@ -173,8 +173,29 @@ __device__ __forceinline__ bool primary_lane(int lane) {
     return (lane & mask) == 0;
 }
 
-// Workgroups are exactly one wavefront; this orders LDS traffic between its
-// lanes (no s_barrier is emitted for a 64-lane workgroup).
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int lane);
+template <>
+__device__ __forceinline__ float readlane_t<float>(float v, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+template <>
+__device__ __forceinline__ double readlane_t<double>(double v, int lane) {
+    uint64_t u = __builtin_bit_cast(uint64_t, v);
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Orders LDS traffic between the lanes of one wavefront.  LDS instructions of
+// a wave execute in order, so no hardware barrier is needed; the fences keep
+// the compiler from moving memory operations across this point, and it must
+// not be an s_barrier: the superleaf kernels run several waves that
+// synchronise independently.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 }  // namespace tsqr
