@@ -56,7 +56,7 @@ def parse():
     p.add_argument("--workload", default="tsqr_max_fp32", choices=sorted(WORKLOADS))
     p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     p.add_argument("--cpu-fraction", type=int, default=8, help="cpu sample = m / this")
-    p.add_argument("--leaf-path", choices=("default", "auto", "householder"), default="default",
+    p.add_argument("--leaf-path", choices=("default", "auto", "householder", "gram_tree"), default="default",
                    help="level-0 superleaf path (default: the library's default for the dtype)")
     return p.parse_args()
 
@@ -118,8 +118,8 @@ def main():
     import torch
 
     import tsqr_amd
-    from tsqr_amd._lib import (TSQR_DOUBLE, TSQR_FLOAT, TSQR_LEAF_AUTO, TSQR_LEAF_HOUSEHOLDER, leaf_stats,
-                               set_leaf_path, timing_read)
+    from tsqr_amd._lib import (TSQR_DOUBLE, TSQR_FLOAT, TSQR_LEAF_AUTO, TSQR_LEAF_GRAM_TREE,
+                               TSQR_LEAF_HOUSEHOLDER, leaf_stats, set_leaf_path, timing_read)
 
     kind, m, n, dtype = WORKLOADS[args.workload]
     tdt = torch.float32 if dtype == "float32" else torch.float64
@@ -139,7 +139,8 @@ def main():
     work = torch.empty(need, dtype=torch.uint8, device=dev)
     fn = getattr(L, f"{kind}_{'s' if code == TSQR_FLOAT else 'd'}_stream")
     if args.leaf_path != "default":
-        set_leaf_path(code, TSQR_LEAF_AUTO if args.leaf_path == "auto" else TSQR_LEAF_HOUSEHOLDER)
+        set_leaf_path(code, {"auto": TSQR_LEAF_AUTO, "householder": TSQR_LEAF_HOUSEHOLDER,
+                             "gram_tree": TSQR_LEAF_GRAM_TREE}[args.leaf_path])
     stream = torch.cuda.current_stream(dev)
 
     def step():
@@ -220,7 +221,7 @@ def main():
             "algorithmic_bytes_per_step": balg,
             "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
             "phase_ms": phase_ms,
-            "superleaves": dict(zip(("gram", "householder"), leaf_stats(work.data_ptr(), m, n, code)))
+            "superleaves": dict(zip(("gram", "householder", "flat"), leaf_stats(work.data_ptr(), m, n, code)))
             if kind == "tsqr" else None,
         },
         "roofline": roofline,

@@ -58,8 +58,9 @@ extern "C" {
 #define TSQR_DOUBLE 2
 
 /* Level-0 leaf paths (tsqr_set_leaf_path). */
-#define TSQR_LEAF_AUTO 0        /* Gram/Cholesky leaves, Householder where rejected */
-#define TSQR_LEAF_HOUSEHOLDER 1 /* Householder leaves only */
+#define TSQR_LEAF_AUTO 0        /* flat Gram path if the whole matrix passes the gate, else GRAM_TREE */
+#define TSQR_LEAF_HOUSEHOLDER 1 /* Householder leaves and tree only */
+#define TSQR_LEAF_GRAM_TREE 2   /* Gram leaves (Householder where rejected), Householder tree */
 
 /* Human-readable text for a status code. */
 const char *tsqr_status_string(int status);
@@ -94,16 +95,21 @@ int qr_s_stream(int m, int n, float *A, int lda, float *R, int ldr, void *work, 
 int qr_d_stream(int m, int n, double *A, int lda, double *R, int ldr, void *work, size_t work_bytes,
                 void *stream);
 
-/* Chooses how level-0 superleaves (8 Householder tiles of rows each) are
- * factored for one dtype, process-wide.  No counterpart in the reference,
- * whose tiles are always Householder (kernelQR.h:69-291).
+/* Chooses how level-0 superleaves (up to 8 Householder tiles of rows each)
+ * are factored for one dtype, process-wide.  No counterpart in the
+ * reference, whose tiles are always Householder (kernelQR.h:69-291).
  *   TSQR_LEAF_HOUSEHOLDER  every tile is a Householder QR and Q is formed
  *                          from the stored reflectors (the reference method);
- *   TSQR_LEAF_AUTO         a superleaf's R comes from the Cholesky factor of
+ *   TSQR_LEAF_GRAM_TREE    a superleaf's R comes from the Cholesky factor of
  *                          its FP64-accumulated Gram matrix and its Q rows
  *                          from A (R^{-1} S) in FP64, unless the Cholesky
  *                          breaks down or ||R||_F ||R^{-1}||_F exceeds gate,
- *                          in which case that superleaf is Householder.
+ *                          in which case that superleaf is Householder; the
+ *                          tree above the superleaves is Householder;
+ *   TSQR_LEAF_AUTO         as GRAM_TREE, except that when the sum of all
+ *                          superleaf Grams itself passes the gate, R is its
+ *                          Cholesky factor and Q = A R^{-1} (FP64), with no
+ *                          tree (the same error bounds).
  * Defaults: FP32 TSQR_LEAF_AUTO with gate 1024, FP64 TSQR_LEAF_HOUSEHOLDER.
  * Below the gate the Gram path's backward error and loss of orthogonality
  * stay at the working roundoff (FP64 accumulation keeps u64*kappa^2 below
@@ -111,9 +117,11 @@ int qr_d_stream(int m, int n, double *A, int lda, double *R, int ldr, void *work
 int tsqr_set_leaf_path(int dtype, int path, double gate);
 
 /* After a tsqr_*_stream call that used `work` has been enqueued: waits for
- * the device and counts the level-0 superleaves that took the Gram path and
- * the Householder path.  Diagnostic; no counterpart in the reference. */
-int tsqr_leaf_stats(const void *work, int m, int n, int dtype, int *gram_leaves, int *householder_leaves);
+ * the device, counts the level-0 superleaves that took the Gram path and the
+ * Householder path, and sets *flat to 1 if the call took the flat Gram path.
+ * Diagnostic; no counterpart in the reference. */
+int tsqr_leaf_stats(const void *work, int m, int n, int dtype, int *gram_leaves, int *householder_leaves,
+                    int *flat);
 
 /* Per-phase timing (no counterpart in the reference, which only times whole
  * calls with CUDA events, test_tsqr.cu:62-90).  While enabled, every call

@@ -85,7 +85,8 @@ def call(kind: str, A: np.ndarray, lda: int | None = None, offset_elems: int = 0
         if st == 0 and kind == "tsqr":
             from tsqr_amd._lib import leaf_stats
 
-            LAST_LEAF_STATS["gram"], LAST_LEAF_STATS["householder"] = leaf_stats(work.data_ptr(), m, n, code)
+            (LAST_LEAF_STATS["gram"], LAST_LEAF_STATS["householder"],
+             LAST_LEAF_STATS["flat"]) = leaf_stats(work.data_ptr(), m, n, code)
     torch.cuda.synchronize()
     Q = from_device_colmajor(buf, m, n, lda, offset_elems)
     R = Rbuf.view(n, n).cpu().numpy().T.copy()

@@ -19,8 +19,9 @@ from tests.gpu_util import LAST_LEAF_STATS, LeafPath, call
 
 pytestmark = pytest.mark.gpu
 
-AUTO, HOUSEHOLDER = 0, 1
-PATHS = [pytest.param(AUTO, id="auto"), pytest.param(HOUSEHOLDER, id="householder")]
+AUTO, HOUSEHOLDER, GRAM_TREE = 0, 1, 2
+PATHS = [pytest.param(AUTO, id="auto"), pytest.param(HOUSEHOLDER, id="householder"),
+         pytest.param(GRAM_TREE, id="gram_tree")]
 
 
 def check(A, Q, R, tag, c=8.0, n_floor=1):
@@ -50,7 +51,7 @@ def test_cfg1_harness_input(dtype, path):
         st, Q, R, _ = call("tsqr", A)
     assert st == 0
     check(A, Q, R, f"cfg1 {np.dtype(dtype).name} path={path}",
-          n_floor=32 if (dtype == np.float64 and path == AUTO) else 1)
+          n_floor=32 if (dtype == np.float64 and path != HOUSEHOLDER) else 1)
     # the reference's restatement at its A800 tuning agrees after sign normalisation
     Qr, Rr = O.tsqr_ref(A, bs=256, num_sm=108)
     _, Rn = O.sign_normalise(None, R)
@@ -81,7 +82,7 @@ def test_shapes(m, n, dtype, path):
     with LeafPath(dtype, path):
         st, Q, R, _ = call("tsqr", A)
     assert st == 0
-    floor = 32 if (dtype == np.float64 and path == AUTO) else 1
+    floor = 32 if (dtype == np.float64 and path != HOUSEHOLDER) else 1
     check(A, Q, R, f"{m}x{n} {np.dtype(dtype).name} path={path} leaves={LAST_LEAF_STATS}", n_floor=floor)
 
 
@@ -147,11 +148,32 @@ def test_reference_shaped_entry_small_ldwork(dtype):
 
 def test_gram_path_is_taken_for_well_conditioned_fp32():
     A = O.gen_normal(200000, 32, np.float32, seed=21)
+    with LeafPath(np.float32, GRAM_TREE):
+        st, Q, R, _ = call("tsqr", A)
+    assert st == 0
+    assert LAST_LEAF_STATS["householder"] == 0 and LAST_LEAF_STATS["gram"] > 1 and not LAST_LEAF_STATS["flat"]
+    check(A, Q, R, f"gram path leaves={LAST_LEAF_STATS}")
+
+
+def test_flat_path_is_taken_for_well_conditioned_fp32():
+    A = O.gen_uniform(200000, 32, np.float32, seed=22)
     with LeafPath(np.float32, AUTO):
         st, Q, R, _ = call("tsqr", A)
     assert st == 0
-    assert LAST_LEAF_STATS["householder"] == 0 and LAST_LEAF_STATS["gram"] > 1
-    check(A, Q, R, f"gram path leaves={LAST_LEAF_STATS}")
+    assert LAST_LEAF_STATS["flat"] == 1
+    check(A, Q, R, f"flat path leaves={LAST_LEAF_STATS}")
+
+
+@pytest.mark.parametrize("cond", [1e2, 5e3])
+def test_flat_gate(cond):
+    """cond 1e2 passes the gate (flat); cond 5e3 gives ||R||_F||R^-1||_F above
+    1024 and must take the tree, with the same error bounds either way."""
+    A = O.gen_conditioned(100000, 32, cond, np.float32, seed=6)
+    with LeafPath(np.float32, AUTO):
+        st, Q, R, _ = call("tsqr", A)
+    assert st == 0
+    assert LAST_LEAF_STATS["flat"] == (1 if cond < 1e3 else 0)
+    check(A, Q, R, f"cond={cond:g} leaves={LAST_LEAF_STATS}")
 
 
 @pytest.mark.parametrize("cond", [1e4, 1e6])
@@ -174,7 +196,7 @@ def test_mixed_leaf_paths_in_one_call():
     leaf = 2048  # FP32, n = 32: 8 tiles of 256 rows
     rows = slice(3 * leaf, 4 * leaf)
     A[rows, 7] = A[rows, 6] * (1 + 1e-6)
-    with LeafPath(np.float32, AUTO):
+    with LeafPath(np.float32, GRAM_TREE):
         st, Q, R, _ = call("tsqr", A)
     assert st == 0
     assert LAST_LEAF_STATS["householder"] == 1 and LAST_LEAF_STATS["gram"] == (m + leaf - 1) // leaf - 1

@@ -12,6 +12,7 @@ TSQR_FLOAT = 1
 TSQR_DOUBLE = 2
 TSQR_LEAF_AUTO = 0
 TSQR_LEAF_HOUSEHOLDER = 1
+TSQR_LEAF_GRAM_TREE = 2
 
 # Every symbol include/tsqr_amd.h declares.
 EXPORTS = (
@@ -69,7 +70,7 @@ def lib() -> ctypes.CDLL:
         f.restype = i32
     L.tsqr_set_leaf_path.argtypes = [i32, i32, ctypes.c_double]
     L.tsqr_set_leaf_path.restype = i32
-    L.tsqr_leaf_stats.argtypes = [vp, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.tsqr_leaf_stats.argtypes = [vp, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.tsqr_leaf_stats.restype = i32
     L.tsqr_timing_enable.argtypes = [i32]
     L.tsqr_timing_enable.restype = i32
@@ -100,12 +101,13 @@ def set_leaf_path(dtype_code: int, path: int, gate: float = 1024.0) -> None:
 
 
 def leaf_stats(work_ptr: int, m: int, n: int, dtype_code: int) -> tuple:
-    """(Gram superleaves, Householder superleaves) of the last call on work."""
-    g, h = ctypes.c_int(0), ctypes.c_int(0)
-    st = lib().tsqr_leaf_stats(work_ptr, m, n, dtype_code, ctypes.byref(g), ctypes.byref(h))
+    """(Gram superleaves, Householder superleaves, flat path taken) of the
+    last call on work."""
+    g, h, f = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    st = lib().tsqr_leaf_stats(work_ptr, m, n, dtype_code, ctypes.byref(g), ctypes.byref(h), ctypes.byref(f))
     if st != TSQR_OK:
         raise TsqrError(st, "tsqr_leaf_stats")
-    return g.value, h.value
+    return g.value, h.value, f.value
 
 
 def status_string(status: int) -> str:

@@ -38,6 +38,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 struct GramLds {
     double G[32][33];
     double row[32];
+    double rdiag[32];  // 1 / R_jj
 };
 
 // Four consecutive rows of one column in the storage type.
@@ -112,18 +113,22 @@ __device__ __forceinline__ void load4(double (&v)[4], const T* __restrict__ base
     }
 }
 
-// Cholesky of L.G (n x n, n <= 32) in place of its upper triangle:
-// L.G[j][c] = R[j][c] for c >= j, R^T R = G.  Columns >= n are treated as an
-// identity block.  Returns false if a pivot is not positive and finite.
-// Lane c < 32 owns column c in g[].
+// Cholesky of L.G (n x n, n <= 32): on return lane c of each half-wave
+// holds column c of R (R^T R = G, positive diagonal) in g[].  The two
+// half-waves (lanes 0-31, 32-63) factor two independent matrices, each in
+// its own GramLds.  Columns >= n are treated as an identity block.  Returns
+// false if a pivot is not positive and finite.
 __device__ __forceinline__ bool chol_wave(GramLds& L, int n, int lane, double (&g)[32]) {
     const int c = lane & 31;
+    const bool hi = lane >= 32;
 #pragma unroll
     for (int r = 0; r < 32; ++r) g[r] = (r < n && c < n) ? L.G[r][c] : (r == c ? 1.0 : 0.0);
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < 32; ++j) {
-        double piv = readlane_t<double>(g[j], j);
+        const double p_lo = readlane_t<double>(g[j], j);
+        const double p_hi = readlane_t<double>(g[j], 32 + j);
+        double piv = hi ? p_hi : p_lo;
         if (!(piv > 0.0) || !(piv < __builtin_inf())) {
             ok = false;
             piv = 1.0;
@@ -131,7 +136,8 @@ __device__ __forceinline__ bool chol_wave(GramLds& L, int n, int lane, double (&
         const double rd = 1.0 / __builtin_sqrt(piv);
         // row j of R: R[j][c] = G[j][c] / sqrt(G[j][j]) for c >= j
         g[j] = (c >= j) ? g[j] * rd : 0.0;
-        if (lane < 32) L.row[c] = g[j];
+        L.row[c] = g[j];
+        if (c == j) L.rdiag[j] = rd;
         wave_sync();
         // G[r][c] -= R[j][r] R[j][c] for j < r <= c
 #pragma unroll
@@ -141,25 +147,25 @@ __device__ __forceinline__ bool chol_wave(GramLds& L, int n, int lane, double (&
     return ok;
 }
 
-// Copies R (lane c < 32 holds column c in g[]) to L.G, row-major.
+// Copies R (lane c of a half-wave holds column c in g[]) to L.G, row-major.
 __device__ __forceinline__ void r_to_lds(const double (&g)[32], GramLds& L, int lane) {
-    if (lane < 32) {
+    const int c = lane & 31;
 #pragma unroll
-        for (int r = 0; r < 32; ++r) L.G[r][lane] = g[r];
-    }
+    for (int r = 0; r < 32; ++r) L.G[r][c] = g[r];
     wave_sync();
 }
 
-// Inverse of the upper triangular R in L.G (row-major): lane c back-
-// substitutes R x = e_c, so x[] is column c of R^{-1}.
+// Inverse of the upper triangular R in L.G (row-major), with 1/R_ii in
+// L.rdiag: lane c back-substitutes R x = e_c, so x[] is column c of R^{-1}.
+// The dot products run as four interleaved partial sums.
 __device__ __forceinline__ void tri_inv_wave(const GramLds& L, int lane, double (&x)[32]) {
     const int c = lane & 31;
 #pragma unroll
     for (int i = 31; i >= 0; --i) {
-        double s = (i == c) ? 1.0 : 0.0;
+        double p[4] = {(i == c) ? 1.0 : 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k = i + 1; k < 32; ++k) s = fmat(-L.G[i][k], x[k], s);
-        x[i] = (i <= c) ? s / L.G[i][i] : 0.0;
+        for (int k = i + 1; k < 32; ++k) p[(k - i) & 3] = fmat(-L.G[i][k], x[k], p[(k - i) & 3]);
+        x[i] = (i <= c) ? ((p[0] + p[1]) + (p[2] + p[3])) * L.rdiag[i] : 0.0;
     }
 }
 
@@ -168,10 +174,13 @@ __device__ __forceinline__ int leaf_rows_of(const LevelDesc& L, int leaf) {
     return (int)(left < L.tile_rows ? left : L.tile_rows);
 }
 
-// Sum over lanes 0..31 (lanes 32..63 hold copies and contribute zero).
-__device__ __forceinline__ double sum32(double v, int lane) {
-    double p[1] = {lane < 32 ? v : 0.0};
-    return wave_sum_transposed<1>(p, lane);
+// Sum over the 32 lanes of each half-wave (lane bits 4..0).
+__device__ __forceinline__ double sum_half(double v) {
+    v = fold_all<1>(v);
+    v = fold_all<2>(v);
+    v = fold_all<3>(v);
+    v = fold_all<4>(v);
+    return fold_all<5>(v);
 }
 
 // Pass 1a of the Gram path, one wavefront per superleaf: G = A_b^T A_b in
@@ -244,31 +253,34 @@ __global__ __launch_bounds__(64) void tsqr_gram_g(const TreeDesc d, const T* __r
     }
 }
 
-// Pass 1b, one wavefront per superleaf: R = chol(G), X = R^{-1},
-// kappa_est = ||R||_F ||R^{-1}||_F.
+// Pass 1b, two superleaves per wavefront (one per half-wave):
+// R = chol(G), X = R^{-1}, kappa_est = ||R||_F ||R^{-1}||_F.
 // Success: R rounded to T goes to the superleaf's slot in its parent block
 // (or to the caller's R if the superleaf is the root) and X (FP64, 32 x 32,
 // row-major) replaces G in side[leaf]; flags[leaf] stays 0.
-// Failure (pivot not positive, or kappa_est > gate): flags[leaf] = 1 and
-// nothing else is written; the Householder superleaf kernels take over.
+// Failure (pivot not positive, or kappa_est > gate): flags[leaf] = 1, the
+// superleaf is appended to list (list[0] counts them, in no particular
+// order), and nothing else is written; the Householder superleaf kernels
+// take over.
 template <typename T>
 __global__ __launch_bounds__(64) void tsqr_gram_chol(const TreeDesc d, T* __restrict__ R, int ldr,
                                                       T* __restrict__ blocks, double* __restrict__ side,
-                                                      int* __restrict__ flags, double gate) {
-    __shared__ GramLds L;
+                                                      int* __restrict__ flags, int* __restrict__ list,
+                                                      const int* __restrict__ ctl, double gate) {
+    __shared__ GramLds Lh[2];
+    if (ctl[0]) return;  // the flat path already has R and R^{-1}
     const int lane = threadIdx.x;
+    const int half = lane >> 5;
+    const int c = lane & 31;
     const int n = d.n;
-    const int leaf = blockIdx.x;
     const LevelDesc& L0 = d.lv[0];
-    double* G = side + (int64_t)leaf * 1024;
-    {
-        const int c = lane & 31;
-        if (lane < 32) {
+    const int leaf = 2 * blockIdx.x + half;
+    const bool real = leaf < L0.tiles;
+    GramLds& L = Lh[half];
+    double* G = side + (int64_t)(real ? leaf : 0) * 1024;
 #pragma unroll 8
-            for (int r = 0; r < 32; ++r) L.G[r][c] = G[r * 32 + c];
-        }
-        wave_sync();
-    }
+    for (int r = 0; r < 32; ++r) L.G[r][c] = real ? G[r * 32 + c] : (r == c ? 1.0 : 0.0);
+    wave_sync();
     double nr;
     bool ok;
     {
@@ -281,7 +293,6 @@ __global__ __launch_bounds__(64) void tsqr_gram_chol(const TreeDesc d, T* __rest
     }
     double x[32];
     tri_inv_wave(L, lane, x);
-    const int c = lane & 31;
     double nx = 0.0;
 #pragma unroll
     for (int r = 0; r < 32; ++r) nx = fmat(x[r], x[r], nx);
@@ -289,10 +300,15 @@ __global__ __launch_bounds__(64) void tsqr_gram_chol(const TreeDesc d, T* __rest
         nr = 0.0;
         nx = 0.0;
     }
-    const double kest = __builtin_sqrt(sum32(nr, lane) * sum32(nx, lane));
+    const double kest = __builtin_sqrt(sum_half(nr) * sum_half(nx));
     ok = ok && (kest <= gate);  // false for NaN too
+    if (!real) return;
     if (!ok) {
-        if (lane == 0) flags[leaf] = 1;
+        if (c == 0) {
+            flags[leaf] = 1;
+            const int at = __hip_atomic_fetch_add(&list[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            list[1 + at] = leaf;
+        }
         return;
     }
     T* dst;
@@ -306,15 +322,13 @@ __global__ __launch_bounds__(64) void tsqr_gram_chol(const TreeDesc d, T* __rest
         dst = blocks + d.lv[1].blk_off + (int64_t)parent * d.H * n + slot * n;
         ld = d.H;
     }
-    if (lane < n) {
+    if (c < n) {
 #pragma unroll 4
-        for (int r = 0; r < n; ++r) dst[r + (int64_t)lane * ld] = (r <= lane) ? (T)L.G[r][lane] : T(0);
+        for (int r = 0; r < n; ++r) dst[r + (int64_t)c * ld] = (r <= c) ? (T)L.G[r][c] : T(0);
     }
     // X row-major: lane c holds column c
-    if (lane < 32) {
 #pragma unroll
-        for (int r = 0; r < 32; ++r) G[r * 32 + c] = x[r];
-    }
+    for (int r = 0; r < 32; ++r) G[r * 32 + c] = x[r];
 }
 
 // Pass 2 of the Gram path, one wavefront per superleaf.
@@ -331,7 +345,7 @@ __global__ __launch_bounds__(64) void tsqr_gram_chol(const TreeDesc d, T* __rest
 template <typename T, bool VEC>
 __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restrict__ A, int lda,
                                                    const double* __restrict__ side, const T* __restrict__ blocks,
-                                                   const int* __restrict__ flags) {
+                                                   const int* __restrict__ flags, const int* __restrict__ ctl) {
     const int leaf = blockIdx.x;
     if (flags[leaf]) return;
     const int lane = threadIdx.x;
@@ -339,8 +353,10 @@ __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restric
     const LevelDesc& L0 = d.lv[0];
     const int li = lane & 15, lq = lane >> 4;
     // ---- M = X S ----
-    const double* X = side + (int64_t)leaf * 1024;  // row-major
-    const bool root = L0.tiles == 1;
+    // flat path: X = R^{-1} of the whole matrix (side block T0) and S = I
+    const bool flat = ctl[0] != 0;
+    const double* X = side + (int64_t)(flat ? L0.tiles : leaf) * 1024;  // row-major
+    const bool root = flat || L0.tiles == 1;
     const T* S = nullptr;
     if (!root) {
         const int parent = leaf / d.arity;
@@ -425,6 +441,82 @@ __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restric
             }
         }
     }
+}
+
+// Flat Gram path, stage 1: partial[p] = sum of the superleaf Grams
+// [p*c, (p+1)*c), c = ceil(T0 / gridDim.x), added in order (deterministic).
+// 256 threads, 4 entries each.  (A template only so that every translation
+// unit that includes this header gets its own copy.)
+template <int UNUSED = 0>
+__global__ __launch_bounds__(256) void tsqr_gram_partial(int leaves, const double* __restrict__ side,
+                                                         double* __restrict__ partial) {
+    const int per = (leaves + gridDim.x - 1) / gridDim.x;
+    const int b0 = blockIdx.x * per;
+    const int b1 = b0 + per < leaves ? b0 + per : leaves;
+    double acc[4] = {0, 0, 0, 0};
+#pragma unroll 4
+    for (int b = b0; b < b1; ++b) {
+        const double* G = side + (int64_t)b * 1024;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] += G[threadIdx.x + 256 * q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) partial[(int64_t)blockIdx.x * 1024 + threadIdx.x + 256 * q] = acc[q];
+}
+
+// Flat Gram path, stage 2 (one workgroup of 1024 threads): G = sum of the
+// partials in order, then wave 0 factors it like a superleaf.  If the
+// Cholesky succeeds and ||R||_F ||R^{-1}||_F <= gate for the whole matrix,
+// R (rounded to T) goes to the caller, R^{-1} to side block T0 and ctl[0] = 1;
+// the per-superleaf kernels and the tree then do nothing and tsqr_gram_q
+// forms Q = A R^{-1}.  Otherwise ctl[0] stays 0 and the tree path runs.
+template <typename T>
+__global__ __launch_bounds__(1024) void tsqr_gram_flat(const TreeDesc d, int nparts,
+                                                        const double* __restrict__ partial, T* __restrict__ R,
+                                                        int ldr, double* __restrict__ side, int* __restrict__ ctl,
+                                                        double gate) {
+    __shared__ GramLds L;
+    const int t = threadIdx.x;
+    const int n = d.n;
+    {
+        double s = 0.0;
+        for (int p = 0; p < nparts; ++p) s += partial[(int64_t)p * 1024 + t];
+        L.G[t >> 5][t & 31] = s;
+    }
+    __syncthreads();
+    if (t >= 64) return;
+    const int lane = t;
+    const int c = lane & 31;
+    double nr;
+    bool ok;
+    {
+        double g[32];
+        ok = chol_wave(L, n, lane, g);  // both half-waves factor the same G
+        nr = 0.0;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) nr = fmat(g[r], g[r], nr);
+        r_to_lds(g, L, lane);
+    }
+    double x[32];
+    tri_inv_wave(L, lane, x);
+    double nx = 0.0;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) nx = fmat(x[r], x[r], nx);
+    if (c >= n) {
+        nr = 0.0;
+        nx = 0.0;
+    }
+    const double kest = __builtin_sqrt(sum_half(nr) * sum_half(nx));
+    ok = ok && (kest <= gate);
+    if (!ok || lane >= 32) return;
+    if (c < n) {
+#pragma unroll 4
+        for (int r = 0; r < n; ++r) R[r + (int64_t)c * ldr] = (r <= c) ? (T)L.G[r][c] : T(0);
+    }
+    double* X = side + (int64_t)d.lv[0].tiles * 1024;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) X[r * 32 + c] = x[r];
+    if (c == 0) ctl[0] = 1;
 }
 
 }  // namespace tsqr

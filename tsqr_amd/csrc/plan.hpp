@@ -4,8 +4,9 @@
 // (kernelQR.h:341-388) inside chunks of NUM_SM*BS rows, and adds a second
 // TSQR over the stacked chunk Rs when m is larger than one chunk
 // (TallShinnyQR.h:106-127).  Here the tree is one structure over all m rows:
-//   level 0   superleaves of leaf_rows = 8*H rows of A (the last one may be
-//             shorter), H = 64*RPL the Householder tile height;
+//   level 0   superleaves of leaf_rows = ST*H rows of A (the last one may be
+//             shorter), H = 64*RPL the Householder tile height and
+//             ST = min(8, H/N) tiles per superleaf;
 //   level l>0 blocks of arity*n rows, arity = floor(H/n); child b of level l
 //             contributes min(rows_b, n) rows of its R at row (b % arity)*n
 //             of block b / arity on level l+1;
@@ -90,9 +91,16 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d) {
 //   blocks   tree blocks of levels >= 1 (elements of T)
 //   tickets  one int per tile of levels >= 1
 //   flags    one int per level-0 superleaf: 0 Gram path, 1 Householder path
-//   side     32 x 32 FP64 per superleaf: R^{-1}, then R^{-1} S (Gram path)
+//   list     count, then the indices of the Householder-path superleaves
+//   ctl      ctl[0] = 1 when the whole matrix took the flat Gram path
+//   partial  min(superleaves, kGramPartials) x 32 x 32 FP64 partial sums of
+//            the superleaf Grams
+//   side     32 x 32 FP64 per superleaf (the Gram matrix, then R^{-1}), and
+//            one more for the whole matrix's R^{-1} on the flat path
+constexpr int kGramPartials = 128;
+
 struct WsLayout {
-    size_t blocks, tickets, flags, side, total;
+    size_t blocks, tickets, flags, list, ctl, partial, side, total;
 };
 
 inline WsLayout ws_layout(const TreeDesc& d, size_t elem) {
@@ -101,8 +109,12 @@ inline WsLayout ws_layout(const TreeDesc& d, size_t elem) {
     w.blocks = 0;
     w.tickets = align256((size_t)d.block_elems * elem);
     w.flags = w.tickets + align256((size_t)d.total_ctr * sizeof(int));
-    w.side = w.flags + align256(leaves * sizeof(int));
-    w.total = w.side + align256(leaves * 1024 * sizeof(double));
+    w.list = w.flags + align256(leaves * sizeof(int));
+    w.ctl = w.list + align256((leaves + 1) * sizeof(int));
+    w.partial = w.ctl + 256;
+    const size_t parts = leaves < (size_t)kGramPartials ? leaves : (size_t)kGramPartials;
+    w.side = w.partial + parts * 1024 * sizeof(double);
+    w.total = w.side + align256((leaves + 1) * 1024 * sizeof(double));
     return w;
 }
 

@@ -50,14 +50,13 @@ __device__ __forceinline__ int super_stack_rows(int rows, int n, int H) {
     return s;
 }
 
+// Both kernels run a grid-stride loop over the Householder-path superleaves
+// listed in list[1 .. list[0]], so they cost one short launch when the Gram
+// path took every superleaf.
 template <typename T, int N, int RPL, bool VEC>
-__global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_r(const TreeDesc d, const T* __restrict__ A, int lda,
-                                                                  T* __restrict__ R, int ldr, T* __restrict__ blocks,
-                                                                  const int* __restrict__ flags, int force) {
+__device__ __forceinline__ void super_r_leaf(const TreeDesc& d, const T* __restrict__ A, int lda, T* __restrict__ R,
+                                             int ldr, T* __restrict__ blocks, SuperLds<T, N, RPL>& S, int leaf) {
     constexpr int H = 64 * RPL;
-    __shared__ SuperLds<T, N, RPL> S;
-    const int leaf = blockIdx.x;
-    if (!force && !flags[leaf]) return;
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int n = d.n;
@@ -87,14 +86,24 @@ __global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_r(const Tre
 }
 
 template <typename T, int N, int RPL, bool VEC>
-__global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_q(const TreeDesc d, T* __restrict__ A, int lda,
-                                                                  const T* __restrict__ blocks,
-                                                                  const int* __restrict__ flags, int force) {
-    constexpr int H = 64 * RPL;
+__global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_r(const TreeDesc d, const T* __restrict__ A,
+                                                                         int lda, T* __restrict__ R, int ldr,
+                                                                         T* __restrict__ blocks,
+                                                                         const int* __restrict__ list) {
     __shared__ SuperLds<T, N, RPL> S;
-    __shared__ T stash[N * H];  // wave 0's Q_tile while it works on the stack
-    const int leaf = blockIdx.x;
-    if (!force && !flags[leaf]) return;
+    const int count = list[0];
+#pragma unroll 1
+    for (int it = blockIdx.x; it < count; it += gridDim.x) {
+        super_r_leaf<T, N, RPL, VEC>(d, A, lda, R, ldr, blocks, S, list[1 + it]);
+        __syncthreads();
+    }
+}
+
+template <typename T, int N, int RPL, bool VEC>
+__device__ __forceinline__ void super_q_leaf(const TreeDesc& d, T* __restrict__ A, int lda,
+                                             const T* __restrict__ blocks, SuperLds<T, N, RPL>& S, T* stash,
+                                             int leaf) {
+    constexpr int H = 64 * RPL;
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int n = d.n;
@@ -123,13 +132,29 @@ __global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_q(const Tre
             const int parent = leaf / d.arity;
             const int slot = leaf - parent * d.arity;
             const T* Sb = blocks + d.lv[1].blk_off + (int64_t)parent * d.H * n + slot * n;
-            apply_s_store<T, N, RPL, true>(y, Sb, d.H, srows < n ? srows : n, n, S.stack, H, srows, lane);
+            apply_s_store<T, N, RPL, true>(y, Sb, d.H, srows < n ? srows : n, n, S.stack, H, srows,
+                                           S.wave[0].R, lane);
         }
         load_rows<T, N, RPL, true>(x, stash, H, H, N, lane);
     }
     __syncthreads();
     if (trows > 0)
-        apply_s_store<T, N, RPL, VEC>(x, S.stack + w * n, H, trows < n ? trows : n, n, a, lda, trows, lane);
+        apply_s_store<T, N, RPL, VEC>(x, S.stack + w * n, H, trows < n ? trows : n, n, a, lda, trows,
+                                      S.wave[w].R, lane);
+}
+
+template <typename T, int N, int RPL, bool VEC>
+__global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_q(const TreeDesc d, T* __restrict__ A, int lda,
+                                                                         const T* __restrict__ blocks,
+                                                                         const int* __restrict__ list) {
+    __shared__ SuperLds<T, N, RPL> S;
+    __shared__ T stash[N * 64 * RPL];  // wave 0's Q_tile while it works on the stack
+    const int count = list[0];
+#pragma unroll 1
+    for (int it = blockIdx.x; it < count; it += gridDim.x) {
+        super_q_leaf<T, N, RPL, VEC>(d, A, lda, blocks, S, stash, list[1 + it]);
+        __syncthreads();
+    }
 }
 
 }  // namespace tsqr

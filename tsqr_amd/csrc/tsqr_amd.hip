@@ -267,7 +267,8 @@ int tsqr_timing_read(double* ms, int* count, int nphases) {
 }
 
 int tsqr_set_leaf_path(int dtype, int path, double gate) {
-    if ((path != TSQR_LEAF_AUTO && path != TSQR_LEAF_HOUSEHOLDER) || !(gate >= 1.0)) return TSQR_ERR_ARG;
+    if ((path != TSQR_LEAF_AUTO && path != TSQR_LEAF_HOUSEHOLDER && path != TSQR_LEAF_GRAM_TREE) || !(gate >= 1.0))
+        return TSQR_ERR_ARG;
     std::lock_guard<std::mutex> lock(g_opts_mu);
     if (dtype == TSQR_FLOAT)
         g_opts_f32 = LaunchOpts{path, gate};
@@ -278,8 +279,9 @@ int tsqr_set_leaf_path(int dtype, int path, double gate) {
     return TSQR_OK;
 }
 
-int tsqr_leaf_stats(const void* work, int m, int n, int dtype, int* gram_leaves, int* householder_leaves) {
-    if (!work || !gram_leaves || !householder_leaves || (dtype != TSQR_FLOAT && dtype != TSQR_DOUBLE))
+int tsqr_leaf_stats(const void* work, int m, int n, int dtype, int* gram_leaves, int* householder_leaves,
+                    int* flat) {
+    if (!work || !gram_leaves || !householder_leaves || !flat || (dtype != TSQR_FLOAT && dtype != TSQR_DOUBLE))
         return TSQR_ERR_ARG;
     if (n < 1 || m < n || n > 32) return TSQR_ERR_SHAPE;
     TreeDesc d;
@@ -288,6 +290,7 @@ int tsqr_leaf_stats(const void* work, int m, int n, int dtype, int* gram_leaves,
     if (!ok) return TSQR_ERR_SHAPE;
     *gram_leaves = 0;
     *householder_leaves = 0;
+    *flat = 0;
     if (d.lv[0].rows <= d.H) {  // single tile: always Householder
         *householder_leaves = 1;
         return TSQR_OK;
@@ -299,6 +302,8 @@ int tsqr_leaf_stats(const void* work, int m, int n, int dtype, int* gram_leaves,
                   hipMemcpyDeviceToHost) != hipSuccess)
         return TSQR_ERR_HIP;
     for (int f : flags) (f ? *householder_leaves : *gram_leaves) += 1;
+    if (hipMemcpy(flat, reinterpret_cast<const char*>(work) + w.ctl, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return TSQR_ERR_HIP;
     return TSQR_OK;
 }
 

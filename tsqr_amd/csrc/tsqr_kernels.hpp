@@ -98,10 +98,30 @@ __device__ __forceinline__ void store_rows(const T (&x)[RPL][N], T* __restrict__
     }
 }
 
-// dst[:, c] = Q_tile[:, 0:kmax] * S[0:kmax, c] for c < n.
+// dst[:, c] = Q_tile[:, 0:kmax] * S[0:kmax, c] for c < n.  S (global memory
+// or LDS, leading dimension ldS) is first copied into the wave's N x N LDS
+// staging area `Ls` with one batch of independent loads (zero beyond kmax
+// rows and n columns), so the column loop reads broadcast LDS instead of
+// waiting on one memory latency per column.
 template <typename T, int N, int RPL, bool VEC>
 __device__ __forceinline__ void apply_s_store(const T (&x)[RPL][N], const T* __restrict__ S, int ldS, int kmax,
-                                              int n, T* __restrict__ dst, int64_t ldd, int rows, int lane) {
+                                              int n, T* __restrict__ dst, int64_t ldd, int rows, T* Ls,
+                                              int lane) {
+    constexpr int PER = (N * N + 63) / 64;
+    T sv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = lane + 64 * q;
+        const int k = idx % N, c = idx / N;
+        sv[q] = (idx < N * N && k < kmax && c < n) ? S[k + (int64_t)c * ldS] : T(0);
+    }
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = lane + 64 * q;
+        if (idx < N * N) Ls[idx] = sv[q];
+    }
+    wave_sync();
     const int row0 = lane * RPL;
 #pragma unroll 1
     for (int c = 0; c < n; ++c) {
@@ -110,14 +130,13 @@ __device__ __forceinline__ void apply_s_store(const T (&x)[RPL][N], const T* __r
         for (int s = 0; s < RPL; ++s) acc[s] = T(0);
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            if (k < kmax) {
-                const T sv = S[k + (int64_t)c * ldS];
+            const T v = Ls[k + c * N];
 #pragma unroll
-                for (int s = 0; s < RPL; ++s) acc[s] = fmat(x[s][k], sv, acc[s]);
-            }
+            for (int s = 0; s < RPL; ++s) acc[s] = fmat(x[s][k], v, acc[s]);
         }
         store_column<T, RPL, VEC>(acc, dst + (int64_t)c * ldd + row0, rows, row0);
     }
+    wave_sync();
 }
 
 // R (n x n, zero below the diagonal) from the wave's LDS to dst (ld),
@@ -144,8 +163,10 @@ __device__ __forceinline__ int tile_rows_of(const LevelDesc& L, int tile) {
 // children) over its block.
 template <typename T, int N, int RPL>
 __global__ __launch_bounds__(64) void tsqr_tree_up(const TreeDesc d, T* __restrict__ R, int ldr,
-                                                    T* __restrict__ blocks, int* __restrict__ tickets) {
+                                                    T* __restrict__ blocks, int* __restrict__ tickets,
+                                                    const int* __restrict__ skip) {
     __shared__ TileLds<T, N> L;
+    if (skip[0]) return;
     const int lane = threadIdx.x;
     const int n = d.n;
     int level = 1;
@@ -205,8 +226,10 @@ __global__ __launch_bounds__(64) void tsqr_single(const TreeDesc d, T* __restric
 }
 
 template <typename T, int N, int RPL>
-__global__ __launch_bounds__(64) void tsqr_down(const TreeDesc d, int level, T* __restrict__ blocks) {
+__global__ __launch_bounds__(64) void tsqr_down(const TreeDesc d, int level, T* __restrict__ blocks,
+                                                 const int* __restrict__ skip) {
     __shared__ TileLds<T, N> L;
+    if (skip[0]) return;
     const int lane = threadIdx.x;
     const int n = d.n;
     const int tile = blockIdx.x;
@@ -220,7 +243,7 @@ __global__ __launch_bounds__(64) void tsqr_down(const TreeDesc d, int level, T* 
     const int parent = tile / d.arity;
     const int slot = tile - parent * d.arity;
     const T* S = blocks + P.blk_off + (int64_t)parent * d.H * n + slot * n;
-    apply_s_store<T, N, RPL, true>(x, S, d.H, rows < n ? rows : n, n, myblk, d.H, rows, lane);
+    apply_s_store<T, N, RPL, true>(x, S, d.H, rows < n ? rows : n, n, myblk, d.H, rows, L.R, lane);
 }
 
 // One copy per translation unit (each dtype's TU launches its own).

@@ -34,8 +34,9 @@ inline bool plan_tsqr(int m, int n, TreeDesc* d) {
 
 // How the level-0 superleaves are factored.
 enum LeafPath {
-    kLeafAuto = 0,         // Gram path, Householder where the gate rejects it
+    kLeafAuto = 0,         // flat Gram path if the whole matrix passes the gate, else kLeafGramTree
     kLeafHouseholder = 1,  // Householder for every superleaf
+    kLeafGramTree = 2,     // Gram path per superleaf (Householder where rejected) + Householder tree
 };
 
 struct LaunchOpts {
