@@ -48,7 +48,7 @@ def test_workspace_sizing(lib):
     assert lib.tsqr_workspace_bytes(100, 33, 1) == 0
     assert lib.tsqr_workspace_bytes(100, 8, 3) == 0
     # single superleaf: no tree, only the per-superleaf flag and FP64 side block
-    assert 0 < lib.tsqr_workspace_bytes(256, 32, 1) <= 16384
+    assert 0 < lib.tsqr_workspace_bytes(256, 32, 1) <= 32768
     # grows about linearly in m and is a small fraction of A (arity 8 for n=32 fp32)
     a = lib.tsqr_workspace_bytes(1 << 20, 32, 1)
     b = lib.tsqr_workspace_bytes(1 << 21, 32, 1)
