@@ -37,6 +37,23 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
+# Measured HBM bytes per dispatch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
+# a 1 GiB copy); the file names the command that produced it.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tsqr_fp32_23887872x32_pmc_traffic.md")
+
+
+def measured_traffic(workload: str):
+    """traffic-json line of the counter summary, if it covers this workload."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic-json:"):
+                    d = json.loads(line.split(":", 1)[1])
+                    return d if d.get("workload") == workload else None
+    except OSError:
+        return None
+    return None
 
 WORKLOADS = {
     # name: (kind, m, n, dtype)
@@ -56,6 +73,9 @@ def parse():
     p.add_argument("--workload", default="tsqr_max_fp32", choices=sorted(WORKLOADS))
     p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     p.add_argument("--cpu-fraction", type=int, default=8, help="cpu sample = m / this")
+    p.add_argument("--copy-bytes", type=int, default=0,
+                   help="before timing, copy a buffer of this many bytes device-to-device (a plain copy of "
+                        "known size for calibrating memory counters) and report its bandwidth")
     p.add_argument("--leaf-path", choices=("default", "auto", "householder", "gram_tree"), default="default",
                    help="level-0 superleaf path (default: the library's default for the dtype)")
     return p.parse_args()
@@ -148,6 +168,22 @@ def main():
         if st != 0:
             raise tsqr_amd.TsqrError(st, "bench step")
 
+    copy_gbs = None
+    if args.copy_bytes > 0:
+        src = torch.empty(args.copy_bytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        copy_gbs = round(2 * src.numel() * 4 * 5 / (e0.elapsed_time(e1) / 1e3) / 1e9, 1)
+        print(f"# copy {src.numel() * 4} B: {copy_gbs} GB/s (read + write)", file=sys.stderr, flush=True)
+        del src, dst
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -182,10 +218,24 @@ def main():
             avg_s = tot_ms / cnt / 1e3
             kern[name] = {"avg_launch_ms": round(avg_s * 1e3, 4), "algorithmic_bytes_per_launch": nbytes,
                           "achieved_GBs": round(nbytes / avg_s / 1e9, 1)}
+    # FP64 MFMA work executed on the Gram path (v_mfma_f64_16x16x4f64, 2048
+    # flops each): 12 per 16 rows in the Gram pass; 64 per 64 rows in the Q
+    # pass, 48 when M = R^-1 is triangular (flat path).
+    stats = leaf_stats(work.data_ptr(), m, n, code) if kind == "tsqr" else (0, 0, 0)
+    gram_rows = m if (kind == "tsqr" and stats[1] == 0) else 0
+    mfma_flops = {"leaf_r": gram_rows / 16 * 12 * 2048,
+                  "leaf_q": gram_rows / 64 * (48 if stats[2] else 64) * 2048}
+    for name in kern:
+        if mfma_flops.get(name):
+            tf = mfma_flops[name] / (kern[name]["avg_launch_ms"] / 1e3) / 1e12
+            kern[name]["fp64_mfma_TFLOPs"] = round(tf, 1)
+            kern[name]["fp64_mfma_frac"] = round(tf / FP64_MFMA_PEAK_TFLOPS, 4)
     roofline = None
     if kern:
         dom = max(kern, key=lambda k: kern[k]["avg_launch_ms"])
         achieved = kern[dom]["achieved_GBs"]
+        wl = f"{kind} m={m} n={n} {dtype}"
+        tr = measured_traffic(wl)
         roofline = {
             "bound": "hbm",
             "kernel": dom,
@@ -193,7 +243,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": tr.get(dom) if tr else None,
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if tr else None,
             "avg_launch_ms": kern[dom]["avg_launch_ms"],
             "algorithmic_bytes_per_launch": kern[dom]["algorithmic_bytes_per_launch"],
             "kernels": kern,
@@ -221,8 +272,8 @@ def main():
             "algorithmic_bytes_per_step": balg,
             "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
             "phase_ms": phase_ms,
-            "superleaves": dict(zip(("gram", "householder", "flat"), leaf_stats(work.data_ptr(), m, n, code)))
-            if kind == "tsqr" else None,
+            "device_copy_GBs": copy_gbs,
+            "superleaves": dict(zip(("gram", "householder", "flat"), stats)) if kind == "tsqr" else None,
         },
         "roofline": roofline,
     }

@@ -201,28 +201,47 @@ __global__ __launch_bounds__(64) void tsqr_gram_g(const TreeDesc d, const T* __r
     const int rq = 4 * (lane >> 4);
     constexpr int D = 4;  // slabs of 16 rows per batch of loads
     int s0 = 0;
-    if (VEC) {
-        // full batches: D*16 rows, no predication
-#pragma unroll 1
-        for (; s0 + D * 16 <= rows; s0 += D * 16) {
-            Raw4<T> a0[D], a1[D];
+    if (VEC && rows >= D * 16) {
+        // Batches of D*16 rows, double buffered: the loads of the next batch
+        // are in flight while the MFMAs consume the current one.
+        Raw4<T> c0[D], c1[D];
 #pragma unroll
-            for (int q = 0; q < D; ++q) {
-                a0[q] = load_raw4<T, true>(base, lda, s0 + 16 * q + rq, col, rows, n);
-                if constexpr (NT == 2) a1[q] = load_raw4<T, true>(base, lda, s0 + 16 * q + rq, col + 16, rows, n);
+        for (int q = 0; q < D; ++q) {
+            c0[q] = load_raw4<T, true>(base, lda, 16 * q + rq, col, rows, n);
+            if constexpr (NT == 2) c1[q] = load_raw4<T, true>(base, lda, 16 * q + rq, col + 16, rows, n);
+        }
+#pragma unroll 1
+        for (;;) {
+            const int next = s0 + D * 16;
+            const bool more = next + D * 16 <= rows;
+            Raw4<T> p0[D], p1[D];
+            if (more) {
+#pragma unroll
+                for (int q = 0; q < D; ++q) {
+                    p0[q] = load_raw4<T, true>(base, lda, next + 16 * q + rq, col, rows, n);
+                    if constexpr (NT == 2)
+                        p1[q] = load_raw4<T, true>(base, lda, next + 16 * q + rq, col + 16, rows, n);
+                }
             }
 #pragma unroll
             for (int q = 0; q < D; ++q) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    const double x0 = (double)a0[q].v[t];
+                    const double x0 = (double)c0[q].v[t];
                     acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, x0, acc00, 0, 0, 0);
                     if constexpr (NT == 2) {
-                        const double x1 = (double)a1[q].v[t];
+                        const double x1 = (double)c1[q].v[t];
                         acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, x1, acc01, 0, 0, 0);
                         acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, x1, acc11, 0, 0, 0);
                     }
                 }
+            }
+            s0 = next;
+            if (!more) break;
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+                c0[q] = p0[q];
+                if constexpr (NT == 2) c1[q] = p1[q];
             }
         }
     }
@@ -415,12 +434,16 @@ __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restric
         v4d acc[4][2];
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = v4d{0, 0, 0, 0};
+        // On the flat path M = R^{-1} is upper triangular: its rows 16..31
+        // are zero in columns 0..15, so those MFMAs are skipped.
 #pragma unroll
         for (int kg = 0; kg < 8; ++kg) {
             if (kg < nk) {
+                const bool lower_zero = flat && kg >= 4;
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg][t], mb[kg][0], acc[t][0], 0, 0, 0);
+                    if (!lower_zero)
+                        acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg][t], mb[kg][0], acc[t][0], 0, 0, 0);
                     acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg][t], mb[kg][1], acc[t][1], 0, 0, 0);
                 }
             }
@@ -464,14 +487,14 @@ __global__ __launch_bounds__(256) void tsqr_gram_partial(int leaves, const doubl
     for (int q = 0; q < 4; ++q) partial[(int64_t)blockIdx.x * 1024 + threadIdx.x + 256 * q] = acc[q];
 }
 
-// Flat Gram path, stage 2 (one workgroup of 1024 threads): G = sum of the
+// Flat Gram path, stage 2 (one workgroup of 256 threads): G = sum of the
 // partials in order, then wave 0 factors it like a superleaf.  If the
 // Cholesky succeeds and ||R||_F ||R^{-1}||_F <= gate for the whole matrix,
 // R (rounded to T) goes to the caller, R^{-1} to side block T0 and ctl[0] = 1;
 // the per-superleaf kernels and the tree then do nothing and tsqr_gram_q
 // forms Q = A R^{-1}.  Otherwise ctl[0] stays 0 and the tree path runs.
 template <typename T>
-__global__ __launch_bounds__(1024) void tsqr_gram_flat(const TreeDesc d, int nparts,
+__global__ __launch_bounds__(256) void tsqr_gram_flat(const TreeDesc d, int nparts,
                                                         const double* __restrict__ partial, T* __restrict__ R,
                                                         int ldr, double* __restrict__ side, int* __restrict__ ctl,
                                                         double gate) {
@@ -479,9 +502,25 @@ __global__ __launch_bounds__(1024) void tsqr_gram_flat(const TreeDesc d, int npa
     const int t = threadIdx.x;
     const int n = d.n;
     {
-        double s = 0.0;
-        for (int p = 0; p < nparts; ++p) s += partial[(int64_t)p * 1024 + t];
-        L.G[t >> 5][t & 31] = s;
+        // thread t owns entries t + 256 e; four chains per entry so that the
+        // loads overlap; a fixed order, so the result is deterministic
+        double s[4][4] = {};
+        int p = 0;
+#pragma unroll 1
+        for (; p + 4 <= nparts; p += 4) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) s[e][q] += partial[(int64_t)(p + q) * 1024 + t + 256 * e];
+        }
+        for (; p < nparts; ++p)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) s[e][0] += partial[(int64_t)p * 1024 + t + 256 * e];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int idx = t + 256 * e;
+            L.G[idx >> 5][idx & 31] = (s[e][0] + s[e][1]) + (s[e][2] + s[e][3]);
+        }
     }
     __syncthreads();
     if (t >= 64) return;

@@ -65,7 +65,7 @@ __device__ __forceinline__ void load_rows(T (&x)[RPL][N], const T* __restrict__ 
     }
 }
 
-template <typename T, int RPL, bool VEC>
+template <typename T, int RPL, bool VEC, bool NT = false>
 __device__ __forceinline__ void store_column(const T (&v)[RPL], T* __restrict__ p, int rows, int row0) {
     using C = Chunk16<T, RPL>;
     if (VEC && row0 + RPL <= rows) {
@@ -74,7 +74,10 @@ __device__ __forceinline__ void store_column(const T (&v)[RPL], T* __restrict__ 
             typename C::vec w;
 #pragma unroll
             for (int e = 0; e < C::kElems; ++e) w[e] = v[q * C::kElems + e];
-            *reinterpret_cast<typename C::vec*>(p + q * C::kElems) = w;
+            if constexpr (NT)
+                __builtin_nontemporal_store(w, reinterpret_cast<typename C::vec*>(p + q * C::kElems));
+            else
+                *reinterpret_cast<typename C::vec*>(p + q * C::kElems) = w;
         }
     } else {
 #pragma unroll

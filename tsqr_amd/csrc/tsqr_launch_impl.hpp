@@ -82,7 +82,7 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
             const int parts = leaves < kGramPartials ? leaves : kGramPartials;
             hipLaunchKernelGGL(tsqr_gram_partial<>, dim3(parts), dim3(256), 0, st, leaves, (const double*)side,
                                partial);
-            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(1024), 0, st, d, parts, (const double*)partial, R,
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, parts, (const double*)partial, R,
                                ldr, side, ctl, o.gate);
         }
         hipLaunchKernelGGL((tsqr_gram_chol<T>), dim3((leaves + 1) / 2), dim3(64), 0, st, d, R, ldr, blocks, side,
