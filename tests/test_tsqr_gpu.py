@@ -188,16 +188,22 @@ def test_ill_conditioned_falls_back_to_householder(cond):
     check(A, Q, R, f"cond={cond:g} leaves={LAST_LEAF_STATS}")
 
 
-def test_mixed_leaf_paths_in_one_call():
-    """One superleaf made nearly rank deficient (two columns almost equal on
-    its rows only) goes Householder, the others stay on the Gram path."""
-    m, n = 40000, 32
+@pytest.mark.parametrize("m", [40000, 4_300_000])
+def test_mixed_leaf_paths_in_one_call(m):
+    """One aligned block of 2048 rows made nearly rank deficient (two columns
+    almost equal on its rows only) sends the superleaves covering it
+    Householder, the others stay on the Gram path.  Superleaves are 1 to 8
+    whole tiles (FP32, n = 32: 256 rows), chosen from m, so the block covers
+    2048 / leaf_rows superleaves exactly: 8 at m = 40000, 1 at m = 4.3e6."""
+    n, tile, block = 32, 256, 2048
     A = O.gen_normal(m, n, np.float32, seed=8)
-    leaf = 2048  # FP32, n = 32: 8 tiles of 256 rows
-    rows = slice(3 * leaf, 4 * leaf)
+    rows = slice(3 * block, 4 * block)
     A[rows, 7] = A[rows, 6] * (1 + 1e-6)
     with LeafPath(np.float32, GRAM_TREE):
         st, Q, R, _ = call("tsqr", A)
     assert st == 0
-    assert LAST_LEAF_STATS["householder"] == 1 and LAST_LEAF_STATS["gram"] == (m + leaf - 1) // leaf - 1
-    check(A, Q, R, f"mixed leaves={LAST_LEAF_STATS}")
+    leaves = LAST_LEAF_STATS["gram"] + LAST_LEAF_STATS["householder"]
+    heights = [t * tile for t in range(1, 9) if -(-m // (t * tile)) == leaves]
+    assert len(heights) == 1, (leaves, heights)
+    assert LAST_LEAF_STATS["householder"] == block // heights[0], (LAST_LEAF_STATS, heights)
+    check(A, Q, R, f"mixed leaves={LAST_LEAF_STATS} leaf_rows={heights[0]}")

@@ -1,0 +1,234 @@
+// gemm_mfma.hpp -- the two coupling products of one recursive QR level on the
+// matrix cores (v_mfma_f32_16x16x4f32 for float, v_mfma_f64_16x16x4f64 for
+// double; products and sums in the working precision, like the reference).
+//
+// The recursive QR (block_QR.h:72-95) couples its column halves by one pass
+// of block classical Gram-Schmidt:
+//   R12 = Q1^T A2   (n1 x n2, reduction over all m rows)   block_QR.h:89-90
+//   A2 -= Q1 R12    (m x n2, inner dimension n1)           block_QR.h:91-92
+// The reference runs both as one thread per output element with a serial k
+// loop (block_QR.h:7-60).
+//
+// Operand layout (both kernels).  A lane loads four consecutive rows of one
+// column with one 16-byte load; MFMA number t (t = 0..3) feeds row 4*q + t
+// (q = lane / 16, or lane % 16 in gemm_nn) as its k index or its row index,
+// so four MFMAs consume 16 rows and every load is a contiguous 16 bytes.
+//
+// gemm_tn_mfma: grid (chunks, ceil(n1/64), ceil(n2/64)), one wave.  Block
+//   (k, bi, bj) sums rows [k*kc, (k+1)*kc) of the 64 x 64 output tile
+//   (bi, bj) into part[k][i + j*n1]; gemm_tn_reduce (below) adds the chunks
+//   in a fixed order, so the result is deterministic.
+// gemm_nn_mfma: grid (ceil(m/64), ceil(n2/64)), one wave.  The 64 x 64 block
+//   of C is loaded into the accumulators, B is streamed 16 rows at a time in
+//   the B-operand layout, and C -= X B is written back.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wave.hpp"
+
+namespace tsqr {
+
+template <typename T>
+struct Mfma16;
+
+template <>
+struct Mfma16<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // accumulator register v of lane (lq = lane/16) holds output row row(v, lq)
+    static __device__ __forceinline__ int row(int v, int lq) { return 4 * lq + v; }
+};
+
+template <>
+struct Mfma16<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int v, int lq) { return 4 * v + lq; }
+};
+
+template <typename T>
+struct Four {
+    T v[4];
+};
+
+// Rows [r, r+4) of column c of X (column-major, ld), zero outside rows < m
+// and columns < ncols; VEC: 16-byte aligned base and ld.
+template <typename T, bool VEC>
+__device__ __forceinline__ Four<T> load_four(const T* __restrict__ X, int64_t ld, int64_t r, int c, int64_t m,
+                                             int ncols) {
+    Four<T> o;
+    if (c < ncols) {
+        const T* p = X + (int64_t)c * ld + r;
+        if (VEC && r + 4 <= m) {
+            if constexpr (sizeof(T) == 4) {
+                typedef float f4 __attribute__((ext_vector_type(4)));
+                const f4 q = *reinterpret_cast<const f4*>(p);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) o.v[t] = q[t];
+            } else {
+                typedef double d2 __attribute__((ext_vector_type(2)));
+                const d2 q0 = *reinterpret_cast<const d2*>(p);
+                const d2 q1 = *reinterpret_cast<const d2*>(p + 2);
+                o.v[0] = q0[0];
+                o.v[1] = q0[1];
+                o.v[2] = q1[0];
+                o.v[3] = q1[1];
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) o.v[t] = (r + t < m) ? p[t] : T(0);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) o.v[t] = T(0);
+    }
+    return o;
+}
+
+template <typename T, bool VEC>
+__device__ __forceinline__ void store_four(const Four<T>& o, T* __restrict__ X, int64_t ld, int64_t r, int c,
+                                           int64_t m, int ncols) {
+    if (c >= ncols) return;
+    T* p = X + (int64_t)c * ld + r;
+    if (VEC && r + 4 <= m) {
+        if constexpr (sizeof(T) == 4) {
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            *reinterpret_cast<f4*>(p) = f4{o.v[0], o.v[1], o.v[2], o.v[3]};
+        } else {
+            typedef double d2 __attribute__((ext_vector_type(2)));
+            *reinterpret_cast<d2*>(p) = d2{o.v[0], o.v[1]};
+            *reinterpret_cast<d2*>(p + 2) = d2{o.v[2], o.v[3]};
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (r + t < m) p[t] = o.v[t];
+    }
+}
+
+// part[chunk] (n1 x n2, ld n1) for the 64 x 64 output tile (blockIdx.y,
+// blockIdx.z) = Q[rows of chunk, cols i]^T A[rows of chunk, cols j].
+template <typename T, bool VEC>
+__global__ __launch_bounds__(64) void gemm_tn_mfma(int m, int n1, int n2, const T* __restrict__ Q, int64_t ldq,
+                                                   const T* __restrict__ A, int64_t lda, int kc,
+                                                   T* __restrict__ part) {
+    using M = Mfma16<T>;
+    const int lane = threadIdx.x;
+    const int li = lane & 15, lq = lane >> 4;
+    const int i0 = blockIdx.y * 64, j0 = blockIdx.z * 64;
+    const int64_t kbeg = (int64_t)blockIdx.x * kc;
+    const int64_t kend = kbeg + kc < m ? kbeg + kc : m;
+    typename M::v4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = typename M::v4{0, 0, 0, 0};
+#pragma unroll 1
+    for (int64_t r0 = kbeg; r0 < kend; r0 += 16) {
+        const int64_t r = r0 + 4 * lq;
+        Four<T> q[4], x[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) q[a] = load_four<T, VEC>(Q, ldq, r, i0 + 16 * a + li, kend, n1);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) x[b] = load_four<T, VEC>(A, lda, r, j0 + 16 * b + li, kend, n2);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = M::mma(q[a].v[t], x[b].v[t], acc[a][b]);
+    }
+    T* out = part + (int64_t)blockIdx.x * n1 * n2;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + 16 * a + M::row(v, lq), j = j0 + 16 * b + li;
+                if (i < n1 && j < n2) out[i + (int64_t)j * n1] = acc[a][b][v];
+            }
+}
+
+// C[i + j*ldc] = sum over chunks of part[k][i + j*n1], chunks in order.
+template <typename T>
+__global__ void gemm_tn_reduce(int n1, int n2, int chunks, const T* __restrict__ part, T* __restrict__ C,
+                               int64_t ldc) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n1 * n2) return;
+    T s = T(0);
+    for (int k = 0; k < chunks; ++k) s += part[(int64_t)k * n1 * n2 + idx];
+    const int i = idx % n1, j = idx / n1;
+    C[i + (int64_t)j * ldc] = s;
+}
+
+// C[rows, j0:j0+64] -= X[rows, 0:K] B[0:K, j0:j0+64] for the 64-row block
+// blockIdx.x and the column group blockIdx.y.  X: m x K (ld ldx), B: K x N
+// (ld ldb), C: m x N (ld ldc), all column-major.
+template <typename T, bool VEC>
+__global__ __launch_bounds__(64) void gemm_nn_mfma(int m, int K, int N, const T* __restrict__ X, int64_t ldx,
+                                                   const T* __restrict__ B, int64_t ldb, T* __restrict__ C,
+                                                   int64_t ldc) {
+    using M = Mfma16<T>;
+    const int lane = threadIdx.x;
+    const int li = lane & 15, lq = lane >> 4;
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    const int j0 = blockIdx.y * 64;
+    // acc[t][jt] register v: row r0 + 4*row(v, lq) + t, column j0 + 16 jt + li
+    typename M::v4 acc[4][4];
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const Four<T> c = load_four<T, VEC>(C, ldc, r0 + 4 * M::row(v, lq), j0 + 16 * jt + li, m, N);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t][jt][v] = c.v[t];
+        }
+#pragma unroll 1
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        // A operand: lane feeds row 4*li + t of column k0 + 4*kg + lq
+        Four<T> xa[4];
+        T bb[4][4];  // B[k0 + 4 kg + lq][j0 + 16 jt + li], negated
+#pragma unroll
+        for (int kg = 0; kg < 4; ++kg) {
+            const int k = k0 + 4 * kg + lq;
+            xa[kg] = load_four<T, VEC>(X, ldx, r0 + 4 * li, k, m, K);
+#pragma unroll
+            for (int jt = 0; jt < 4; ++jt) {
+                const int j = j0 + 16 * jt + li;
+                bb[kg][jt] = (k < K && j < N) ? -B[k + (int64_t)j * ldb] : T(0);
+            }
+        }
+#pragma unroll
+        for (int kg = 0; kg < 4; ++kg)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int jt = 0; jt < 4; ++jt) acc[t][jt] = M::mma(xa[kg].v[t], bb[kg][jt], acc[t][jt]);
+    }
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            Four<T> c;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) c.v[t] = acc[t][jt][v];
+            store_four<T, VEC>(c, C, ldc, r0 + 4 * M::row(v, lq), j0 + 16 * jt + li, m, N);
+        }
+}
+
+// Zeroes rows [r0, r0+nr) x columns [c0, c0+nc) of a column-major matrix.
+template <typename T>
+__global__ void zero_block(T* __restrict__ M, int64_t ld, int r0, int nr, int c0, int nc) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nr * nc) return;
+    const int r = idx % nr, c = idx / nr;
+    M[(r0 + r) + (int64_t)(c0 + c) * ld] = T(0);
+}
+
+}  // namespace tsqr

@@ -1,7 +1,9 @@
 // gram.hpp -- the Gram (CholeskyQR) path for well-conditioned superleaves.
 //
-// A superleaf is a block of super_tiles*H consecutive rows of A (H = tile
-// height of the Householder kernels, super_tiles = min(8, H/N)).  For such a block A_b the R factor is unique up to
+// A superleaf is a block of between one and super_tiles tiles of H
+// consecutive rows of A (H = tile height of the Householder kernels,
+// super_tiles = min(8, H/N); the height is chosen from m by leaf_rows_for).
+// For such a block A_b the R factor is unique up to
 // the signs of its rows, so it can be obtained either by Householder
 // reflections (super.hpp) or from the Gram matrix:
 //   G_b = A_b^T A_b   accumulated in FP64 on the matrix cores

@@ -4,9 +4,9 @@
 // (kernelQR.h:341-388) inside chunks of NUM_SM*BS rows, and adds a second
 // TSQR over the stacked chunk Rs when m is larger than one chunk
 // (TallShinnyQR.h:106-127).  Here the tree is one structure over all m rows:
-//   level 0   superleaves of leaf_rows = ST*H rows of A (the last one may be
-//             shorter), H = 64*RPL the Householder tile height and
-//             ST = min(8, H/N) tiles per superleaf;
+//   level 0   superleaves of leaf_rows rows of A (the last one may be
+//             shorter): between 1 and ST tiles of H rows, H = 64*RPL the
+//             Householder tile height and ST = min(8, H/N);
 //   level l>0 blocks of arity*n rows, arity = floor(H/n); child b of level l
 //             contributes min(rows_b, n) rows of its R at row (b % arity)*n
 //             of block b / arity on level l+1;

@@ -4,7 +4,7 @@
 //          (tsqr_kernels.hpp) are compiled per dtype in tsqr_f32.hip and
 //          tsqr_f64.hip.
 // qr_*   : the reference's recursive column split (block_QR.h:72-95) with
-//          the coupling products of gemm_valu.hpp.
+//          the MFMA coupling products of gemm_mfma.hpp.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -12,7 +12,7 @@
 #include <vector>
 
 #include "../../include/tsqr_amd.h"
-#include "gemm_valu.hpp"
+#include "gemm_mfma.hpp"
 #include "tsqr_launch.hpp"
 
 namespace tsqr {
@@ -135,10 +135,25 @@ int tsqr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t w
 // a multiple of 32 in general.
 inline int split_cols(int n) { return 32 * ((n + 63) / 64); }
 
-inline int tn_chunk_rows(int m) {
-    int kc = (m + 1023) / 1024;
-    kc = (kc + 31) / 32 * 32;
-    return kc < 1024 ? 1024 : kc;
+// Rows per split-K chunk of R12 = Q1^T A2: enough chunks that the launch has
+// about 512 waves, each chunk a multiple of 16 rows and at least 64 rows.
+inline int tn_chunk_rows(int m, int n1, int n2) {
+    const int tiles = ((n1 + 63) / 64) * ((n2 + 63) / 64);
+    const int chunks = (512 + tiles - 1) / tiles;
+    int kc = (m + chunks - 1) / chunks;
+    kc = (kc + 15) / 16 * 16;
+    return kc < 64 ? 64 : kc;
+}
+
+// Largest chunks * n1 * n2 over the recursion (the partial sums of R12).
+inline size_t tn_partial_elems(int m, int n) {
+    if (n <= 32) return 0;
+    const int n1 = split_cols(n), n2 = n - n1;
+    const int kc = tn_chunk_rows(m, n1, n2);
+    const size_t here = (size_t)((m + kc - 1) / kc) * n1 * n2;
+    const size_t a = tn_partial_elems(m, n1), b = tn_partial_elems(m, n2);
+    size_t mx = here > a ? here : a;
+    return mx > b ? mx : b;
 }
 
 template <typename T>
@@ -146,10 +161,7 @@ size_t qr_ws_bytes(int m, int n) {
     if (n < 1 || m < n) return 0;
     if (n <= 32) return tsqr_ws_bytes<T>(m, n);
     const size_t panel = tsqr_ws_bytes<T>(m, 32);
-    const int n1 = split_cols(n), n2 = n - n1;
-    const int kc = tn_chunk_rows(m);
-    const size_t chunks = (size_t)((m + kc - 1) / kc);
-    return align256(panel) + align256(chunks * (size_t)n1 * n2 * sizeof(T));
+    return align256(panel) + align256(tn_partial_elems(m, n) * sizeof(T));
 }
 
 template <typename T>
@@ -163,18 +175,27 @@ int qr_rec(int m, int n, T* A, int lda, T* R, int ldr, char* ws, size_t panel_by
     T* R22 = R + (int64_t)n1 * ldr + n1;
     int e = qr_rec<T>(m, n1, A1, lda, R11, ldr, ws, panel_bytes, part, st);
     if (e) return e;
-    const int kc = tn_chunk_rows(m);
+    const int kc = tn_chunk_rows(m, n1, n2);
     const int chunks = (m + kc - 1) / kc;
     const bool timed = timing_enabled();
+    const bool vec = ((uintptr_t)A % 16 == 0) && (((size_t)lda * sizeof(T)) % 16 == 0);
     if (timed) timing_mark(kPhaseGemmTN, true, st);
-    hipLaunchKernelGGL(gemm_tn_partial<T>, dim3(chunks, (n1 + 63) / 64, (n2 + 63) / 64), dim3(256), 0, st, m, n1, n2,
-                       A1, (int64_t)lda, A2, (int64_t)lda, kc, part);
+    if (vec)
+        hipLaunchKernelGGL((gemm_tn_mfma<T, true>), dim3(chunks, (n1 + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m,
+                           n1, n2, (const T*)A1, (int64_t)lda, (const T*)A2, (int64_t)lda, kc, part);
+    else
+        hipLaunchKernelGGL((gemm_tn_mfma<T, false>), dim3(chunks, (n1 + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m,
+                           n1, n2, (const T*)A1, (int64_t)lda, (const T*)A2, (int64_t)lda, kc, part);
     hipLaunchKernelGGL(gemm_tn_reduce<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, n1, n2, chunks, part, R12,
                        (int64_t)ldr);
     if (timed) timing_mark(kPhaseGemmTN, false, st);
     if (timed) timing_mark(kPhaseGemmNN, true, st);
-    hipLaunchKernelGGL(gemm_nn_update<T>, dim3((m + 255) / 256, (n2 + 63) / 64), dim3(256), 0, st, m, n1, n2, A1,
-                       (int64_t)lda, R12, (int64_t)ldr, A2, (int64_t)lda);
+    if (vec)
+        hipLaunchKernelGGL((gemm_nn_mfma<T, true>), dim3((m + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m, n1, n2,
+                           (const T*)A1, (int64_t)lda, (const T*)R12, (int64_t)ldr, A2, (int64_t)lda);
+    else
+        hipLaunchKernelGGL((gemm_nn_mfma<T, false>), dim3((m + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m, n1, n2,
+                           (const T*)A1, (int64_t)lda, (const T*)R12, (int64_t)ldr, A2, (int64_t)lda);
     if (timed) timing_mark(kPhaseGemmNN, false, st);
     hipLaunchKernelGGL(zero_block<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, R, (int64_t)ldr, n1, n2, 0, n1);
     if (hipGetLastError() != hipSuccess) return TSQR_ERR_HIP;

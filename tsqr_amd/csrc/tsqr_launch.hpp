@@ -25,11 +25,29 @@ constexpr int super_tiles(int N) {
     return (64 * rows_per_lane<T>(N)) / N < 8 ? (64 * rows_per_lane<T>(N)) / N : 8;
 }
 
+// Waves the level-0 passes aim to keep busy: two per SIMD of a 256-CU part.
+constexpr int kTargetLeaves = 2048;
+
+// Superleaf height: about m / kTargetLeaves rows, a whole number of tiles,
+// between one tile and super_tiles tiles.  Large m gets full superleaves
+// (fewer, longer streams and a smaller tree); small m gets one-tile
+// superleaves so that the level-0 passes still fill the GPU.
+template <typename T>
+inline int leaf_rows_for(int m, int n) {
+    const int N = padded_cols(n);
+    const int H = 64 * rows_per_lane<T>(N);
+    const int64_t want = ((int64_t)m + kTargetLeaves - 1) / kTargetLeaves;
+    int64_t tiles = (want + H - 1) / H;
+    if (tiles < 1) tiles = 1;
+    if (tiles > super_tiles<T>(N)) tiles = super_tiles<T>(N);
+    return (int)tiles * H;
+}
+
 template <typename T>
 inline bool plan_tsqr(int m, int n, TreeDesc* d) {
     const int N = padded_cols(n);
     const int H = 64 * rows_per_lane<T>(N);
-    return make_tree(m, n, H, super_tiles<T>(N) * H, d);
+    return make_tree(m, n, H, leaf_rows_for<T>(m, n), d);
 }
 
 // How the level-0 superleaves are factored.
