@@ -38,6 +38,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 # Measured HBM bytes per dispatch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
 # a 1 GiB copy); the file names the command that produced it.
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tsqr_fp32_23887872x32_pmc_traffic.md")
@@ -62,6 +63,8 @@ WORKLOADS = {
     "tsqr_27648x32_fp32": ("tsqr", 27648, 32, "float32"),
     "qr_27648x128_fp32": ("qr", 27648, 128, "float32"),
     "qr_27648x128_fp64": ("qr", 27648, 128, "float64"),
+    "tsqr_mixed_max_fp32": ("tsqr_mixed", 23887872, 32, "float32"),
+    "tsqr_mixed_27648x32_fp32": ("tsqr_mixed", 27648, 32, "float32"),
 }
 
 
@@ -139,7 +142,7 @@ def main():
 
     import tsqr_amd
     from tsqr_amd._lib import (TSQR_DOUBLE, TSQR_FLOAT, TSQR_LEAF_AUTO, TSQR_LEAF_GRAM_TREE,
-                               TSQR_LEAF_HOUSEHOLDER, leaf_stats, set_leaf_path, timing_read)
+                               TSQR_LEAF_HOUSEHOLDER, leaf_stats, mixed_stats, set_leaf_path, timing_read)
 
     kind, m, n, dtype = WORKLOADS[args.workload]
     tdt = torch.float32 if dtype == "float32" else torch.float64
@@ -155,9 +158,10 @@ def main():
     A.uniform_(-1.0, 1.0, generator=g)  # column-major m x n with lda = m
     R = torch.zeros((n, n), dtype=tdt, device=dev)
     L = tsqr_amd.lib()
-    need = (L.tsqr_workspace_bytes if kind == "tsqr" else L.qr_workspace_bytes)(m, n, code)
+    need = (L.qr_workspace_bytes if kind == "qr" else L.tsqr_workspace_bytes)(m, n, code)
     work = torch.empty(need, dtype=torch.uint8, device=dev)
     fn = getattr(L, f"{kind}_{'s' if code == TSQR_FLOAT else 'd'}_stream")
+    is_tsqr = kind in ("tsqr", "tsqr_mixed")
     if args.leaf_path != "default":
         set_leaf_path(code, {"auto": TSQR_LEAF_AUTO, "householder": TSQR_LEAF_HOUSEHOLDER,
                              "gram_tree": TSQR_LEAF_GRAM_TREE}[args.leaf_path])
@@ -200,7 +204,7 @@ def main():
     print(f"# phases (ms total, count): {phases}", file=sys.stderr, flush=True)
 
     sec = (t1 - t0) / args.steps
-    balg = (3 * m * n + n * n) * esz if kind == "tsqr" else None
+    balg = (3 * m * n + n * n) * esz if is_tsqr else None
     if kind == "qr":
         # n = 32*2^k: every column block is read twice and written once by its
         # panel TSQR, plus the coupling products' traffic; report the panel rate
@@ -209,7 +213,7 @@ def main():
 
     # Algorithmic bytes per launch of the two streaming kernels: the leaf R
     # pass reads A (m*n elements), the leaf Q pass reads A and writes Q.
-    panel_n = n if kind == "tsqr" else 32
+    panel_n = n if is_tsqr else 32
     alg = {"leaf_r": m * panel_n * esz, "leaf_q": 2 * m * panel_n * esz}
     kern = {}
     for name, nbytes in alg.items():
@@ -218,18 +222,37 @@ def main():
             avg_s = tot_ms / cnt / 1e3
             kern[name] = {"avg_launch_ms": round(avg_s * 1e3, 4), "algorithmic_bytes_per_launch": nbytes,
                           "achieved_GBs": round(nbytes / avg_s / 1e9, 1)}
-    # FP64 MFMA work executed on the Gram path (v_mfma_f64_16x16x4f64, 2048
-    # flops each): 12 per 16 rows in the Gram pass; 64 per 64 rows in the Q
-    # pass, 48 when M = R^-1 is triangular (flat path).
-    stats = leaf_stats(work.data_ptr(), m, n, code) if kind == "tsqr" else (0, 0, 0)
-    gram_rows = m if (kind == "tsqr" and stats[1] == 0) else 0
-    mfma_flops = {"leaf_r": gram_rows / 16 * 12 * 2048,
-                  "leaf_q": gram_rows / 64 * (48 if stats[2] else 64) * 2048}
+    # Matrix-core work executed per launch.  FP64 (v_mfma_f64_16x16x4f64,
+    # 2048 flops each): Gram pass 12 per 16 rows; Q pass 64 per 64 rows, 48
+    # when M = R^-1 is triangular (flat and mixed paths).  BF16
+    # (v_mfma_f32_16x16x32_bf16, 16384 flops each, mixed path): six per 16x16
+    # block per 32 rows, three blocks for n > 16, in the Gram pass and again
+    # for the residual check fused into the Q pass.
+    mixed = None
+    if kind == "tsqr":
+        stats = leaf_stats(work.data_ptr(), m, n, code)
+        gram_rows = m if stats[1] == 0 else 0
+        f64 = {"leaf_r": gram_rows / 16 * 12 * 2048, "leaf_q": gram_rows / 64 * (48 if stats[2] else 64) * 2048}
+        bf16 = {}
+    elif kind == "tsqr_mixed":
+        mixed = mixed_stats(work.data_ptr(), m, n)
+        stats = (0, 0, mixed["flat"])
+        rows = m if mixed["flat"] else 0
+        blocks = 3 if n > 16 else 1
+        f64 = {"leaf_q": rows / 64 * 48 * 2048 * mixed["passes"]}
+        bf16 = {"leaf_r": rows / 32 * 6 * blocks * 16384, "leaf_q": rows / 32 * 6 * blocks * 16384 * mixed["passes"]}
+    else:
+        stats, f64, bf16 = (0, 0, 0), {}, {}
     for name in kern:
-        if mfma_flops.get(name):
-            tf = mfma_flops[name] / (kern[name]["avg_launch_ms"] / 1e3) / 1e12
+        secs = kern[name]["avg_launch_ms"] / 1e3
+        if f64.get(name):
+            tf = f64[name] / secs / 1e12
             kern[name]["fp64_mfma_TFLOPs"] = round(tf, 1)
             kern[name]["fp64_mfma_frac"] = round(tf / FP64_MFMA_PEAK_TFLOPS, 4)
+        if bf16.get(name):
+            tf = bf16[name] / secs / 1e12
+            kern[name]["bf16_mfma_TFLOPs"] = round(tf, 1)
+            kern[name]["bf16_mfma_frac"] = round(tf / BF16_MFMA_PEAK_TFLOPS, 4)
     roofline = None
     if kern:
         dom = max(kern, key=lambda k: kern[k]["avg_launch_ms"])
@@ -274,6 +297,7 @@ def main():
             "phase_ms": phase_ms,
             "device_copy_GBs": copy_gbs,
             "superleaves": dict(zip(("gram", "householder", "flat"), stats)) if kind == "tsqr" else None,
+            "mixed": mixed,
         },
         "roofline": roofline,
     }

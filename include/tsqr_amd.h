@@ -95,6 +95,31 @@ int qr_s_stream(int m, int n, float *A, int lda, float *R, int ldr, void *work, 
 int qr_d_stream(int m, int n, double *A, int lda, double *R, int ldr, void *work, size_t work_bytes,
                 void *stream);
 
+/* Mixed-precision TSQR (FP32 in and out, n <= 32), the tsqr<float> contract
+ * (TallShinnyQR.h:86-149) with a different method; no counterpart in the
+ * reference.  The Gram matrices are formed on the BF16 matrix cores with
+ * exact three-way BF16 splits of the FP32 data (six products per pair, FP32
+ * accumulation over 128 rows, FP64 beyond), R = chol(A^T A) and
+ * Q = A R^{-1} (FP64 products, one rounding), followed by an FP32 residual
+ * check ||Q^T Q - I||_F <= tol computed in the same pass; if the check fails
+ * a second CholeskyQR pass (CholeskyQR2) re-orthogonalises Q and updates R.
+ * If the Cholesky breaks down or ||R||_F ||R^{-1}||_F exceeds the gate, the
+ * call runs the Householder tree instead.  Same workspace as tsqr_s. */
+int tsqr_mixed_s(int m, int n, float *A, int lda, float *R, int ldr, float *work, int ldwork);
+int tsqr_mixed_s_stream(int m, int n, float *A, int lda, float *R, int ldr, void *work, size_t work_bytes,
+                        void *stream);
+
+/* Gate (>= 1, default 1024) and residual tolerance (> 0, default 8e-7) of
+ * the mixed entry points, process-wide. */
+int tsqr_set_mixed(double gate, double tol);
+
+/* After a tsqr_mixed_s* call on the same stream, with its workspace: *flat =
+ * 1 if the CholeskyQR path ran (0: Householder tree), *passes = 1 or 2,
+ * *check = 0 if the final residual met tol (1: it did not after the second
+ * pass, 2: the second Cholesky broke down), *residual = the final
+ * ||Q^T Q - I||_F estimate.  Synchronises the device.  Diagnostic. */
+int tsqr_mixed_stats(const void *work, int m, int n, int *flat, int *passes, int *check, double *residual);
+
 /* Chooses how level-0 superleaves (up to 8 Householder tiles of rows each)
  * are factored for one dtype, process-wide.  No counterpart in the
  * reference, whose tiles are always Householder (kernelQR.h:69-291).

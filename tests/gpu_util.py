@@ -55,13 +55,15 @@ class LeafPath:
 
 
 LAST_LEAF_STATS = {}
+LAST_MIXED_STATS = {}
 
 
 def call(kind: str, A: np.ndarray, lda: int | None = None, offset_elems: int = 0, work_bytes: int | None = None,
          ref_shaped: bool = False, ldwork: int = 0):
-    """Runs tsqr/qr on a copy of A through the C ABI.  Returns (status, Q, R,
-    padding_untouched); for tsqr through the stream entry point the Gram /
-    Householder superleaf counts land in LAST_LEAF_STATS."""
+    """Runs tsqr / qr / tsqr_mixed (kind) on a copy of A through the C ABI.
+    Returns (status, Q, R, padding_untouched); through the stream entry
+    points the Gram / Householder superleaf counts of tsqr land in
+    LAST_LEAF_STATS and the outcome of tsqr_mixed in LAST_MIXED_STATS."""
     import torch
 
     import tsqr_amd
@@ -73,7 +75,7 @@ def call(kind: str, A: np.ndarray, lda: int | None = None, offset_elems: int = 0
     suf = "s" if code == TSQR_FLOAT else "d"
     buf, pA, lda = to_device_colmajor(A, lda, offset_elems)
     Rbuf = torch.full((n * n,), float("nan"), dtype=torch_dtype(A.dtype), device="cuda")
-    need = (L.tsqr_workspace_bytes if kind == "tsqr" else L.qr_workspace_bytes)(m, n, code)
+    need = (L.qr_workspace_bytes if kind == "qr" else L.tsqr_workspace_bytes)(m, n, code)
     if ref_shaped:
         elem = 4 if code == TSQR_FLOAT else 8
         work = torch.empty(max(ldwork * n * elem, 1), dtype=torch.uint8, device="cuda")
@@ -87,6 +89,11 @@ def call(kind: str, A: np.ndarray, lda: int | None = None, offset_elems: int = 0
 
             (LAST_LEAF_STATS["gram"], LAST_LEAF_STATS["householder"],
              LAST_LEAF_STATS["flat"]) = leaf_stats(work.data_ptr(), m, n, code)
+        if st == 0 and kind == "tsqr_mixed":
+            from tsqr_amd._lib import mixed_stats
+
+            LAST_MIXED_STATS.clear()
+            LAST_MIXED_STATS.update(mixed_stats(work.data_ptr(), m, n))
     torch.cuda.synchronize()
     Q = from_device_colmajor(buf, m, n, lda, offset_elems)
     R = Rbuf.view(n, n).cpu().numpy().T.copy()

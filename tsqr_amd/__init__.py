@@ -7,6 +7,10 @@ that takes torch tensors only as holders of device memory and streams:
     tsqr(A, R=None, work=None)   A (m x n, column-major, on the GPU) is
                                  overwritten by Q; returns (A, R)
     qr(A, R=None, work=None)     recursive blocked QR, same convention
+    tsqr_mixed(A, R=None, work=None)
+                                 float32 TSQR on the BF16 matrix cores
+                                 (CholeskyQR with a residual check and an
+                                 optional second pass), same convention
 
 A column-major m x n tensor is one whose strides are (1, lda), for example
 ``colmajor(torch.randn(m, n))``.  Errors from the library raise TsqrError
@@ -24,7 +28,7 @@ from ._lib import (  # noqa: F401
     qr_workspace_bytes,
     tsqr_workspace_bytes,
 )
-from .api import colmajor, empty_colmajor, qr, tsqr  # noqa: F401
+from .api import colmajor, empty_colmajor, qr, tsqr, tsqr_mixed  # noqa: F401
 
 __all__ = [
     "LIB_PATH",
@@ -35,5 +39,6 @@ __all__ = [
     "qr",
     "qr_workspace_bytes",
     "tsqr",
+    "tsqr_mixed",
     "tsqr_workspace_bytes",
 ]

@@ -31,6 +31,10 @@ EXPORTS = (
     "tsqr_leaf_stats",
     "tsqr_timing_enable",
     "tsqr_timing_read",
+    "tsqr_mixed_s",
+    "tsqr_mixed_s_stream",
+    "tsqr_set_mixed",
+    "tsqr_mixed_stats",
 )
 
 
@@ -60,11 +64,11 @@ def lib() -> ctypes.CDLL:
     for f in (L.tsqr_workspace_bytes, L.qr_workspace_bytes):
         f.argtypes = [i32, i32, i32]
         f.restype = sz
-    for name in ("tsqr_s", "tsqr_d", "qr_s", "qr_d"):
+    for name in ("tsqr_s", "tsqr_d", "qr_s", "qr_d", "tsqr_mixed_s"):
         f = getattr(L, name)
         f.argtypes = [i32, i32, vp, i32, vp, i32, vp, i32]
         f.restype = i32
-    for name in ("tsqr_s_stream", "tsqr_d_stream", "qr_s_stream", "qr_d_stream"):
+    for name in ("tsqr_s_stream", "tsqr_d_stream", "qr_s_stream", "qr_d_stream", "tsqr_mixed_s_stream"):
         f = getattr(L, name)
         f.argtypes = [i32, i32, vp, i32, vp, i32, vp, sz, vp]
         f.restype = i32
@@ -72,6 +76,11 @@ def lib() -> ctypes.CDLL:
     L.tsqr_set_leaf_path.restype = i32
     L.tsqr_leaf_stats.argtypes = [vp, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.tsqr_leaf_stats.restype = i32
+    L.tsqr_set_mixed.argtypes = [ctypes.c_double, ctypes.c_double]
+    L.tsqr_set_mixed.restype = i32
+    pi = ctypes.POINTER(i32)
+    L.tsqr_mixed_stats.argtypes = [vp, i32, i32, pi, pi, pi, ctypes.POINTER(ctypes.c_double)]
+    L.tsqr_mixed_stats.restype = i32
     L.tsqr_timing_enable.argtypes = [i32]
     L.tsqr_timing_enable.restype = i32
     L.tsqr_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), i32]
@@ -108,6 +117,23 @@ def leaf_stats(work_ptr: int, m: int, n: int, dtype_code: int) -> tuple:
     if st != TSQR_OK:
         raise TsqrError(st, "tsqr_leaf_stats")
     return g.value, h.value, f.value
+
+
+def set_mixed(gate: float = 1024.0, tol: float = 8e-7) -> None:
+    st = lib().tsqr_set_mixed(gate, tol)
+    if st != TSQR_OK:
+        raise TsqrError(st, "tsqr_set_mixed")
+
+
+def mixed_stats(work_ptr: int, m: int, n: int) -> dict:
+    """Outcome of the last tsqr_mixed_s* call on work: CholeskyQR path taken
+    (flat), passes, check status and the final ||Q^T Q - I||_F estimate."""
+    f, p, c = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    r = ctypes.c_double(0.0)
+    st = lib().tsqr_mixed_stats(work_ptr, m, n, ctypes.byref(f), ctypes.byref(p), ctypes.byref(c), ctypes.byref(r))
+    if st != TSQR_OK:
+        raise TsqrError(st, "tsqr_mixed_stats")
+    return {"flat": f.value, "passes": p.value, "check": c.value, "residual": r.value}
 
 
 def status_string(status: int) -> str:

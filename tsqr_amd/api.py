@@ -36,13 +36,17 @@ def _run(kind: str, A: torch.Tensor, R, work, stream):
     if R is None:
         R = empty_colmajor(n, n, A.dtype, A.device)
     ldr = _ld(R, "R")
-    need = (tsqr_workspace_bytes if kind == "tsqr" else qr_workspace_bytes)(m, n, code)
+    need = (qr_workspace_bytes if kind == "qr" else tsqr_workspace_bytes)(m, n, code)
     if work is None:
         work = torch.empty(max(need, 1), dtype=torch.uint8, device=A.device)
     nbytes = work.numel() * work.element_size()
     if stream is None:
         stream = torch.cuda.current_stream(A.device)
     suf = "s" if code == TSQR_FLOAT else "d"
+    if kind == "tsqr_mixed":
+        if code != TSQR_FLOAT:
+            raise TypeError("the mixed-precision TSQR takes float32 input")
+        suf = "s"
     fn = getattr(lib(), f"{kind}_{suf}_stream")
     st = fn(m, n, A.data_ptr(), lda, R.data_ptr(), ldr, work.data_ptr(), nbytes, stream.cuda_stream)
     if st != 0:
@@ -59,3 +63,10 @@ def tsqr(A: torch.Tensor, R: torch.Tensor | None = None, work: torch.Tensor | No
 def qr(A: torch.Tensor, R: torch.Tensor | None = None, work: torch.Tensor | None = None, stream=None):
     """Recursive blocked QR (block_QR.h:72-95); same conventions as tsqr."""
     return _run("qr", A, R, work, stream)
+
+
+def tsqr_mixed(A: torch.Tensor, R: torch.Tensor | None = None, work: torch.Tensor | None = None, stream=None):
+    """Mixed-precision TSQR of a float32 column-major m x n GPU tensor, n <= 32:
+    BF16x3 CholeskyQR with an FP32 residual check and an optional second pass.
+    Same conventions as tsqr."""
+    return _run("tsqr_mixed", A, R, work, stream)

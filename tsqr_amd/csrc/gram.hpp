@@ -468,13 +468,16 @@ __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restric
     }
 }
 
-// Flat Gram path, stage 1: partial[p] = sum of the superleaf Grams
-// [p*c, (p+1)*c), c = ceil(T0 / gridDim.x), added in order (deterministic).
-// 256 threads, 4 entries each.  (A template only so that every translation
+// Flat Gram path, stage 1: partial[p] = sum of the 32 x 32 blocks
+// [p*c, (p+1)*c) of side, c = ceil(leaves / gridDim.x), added in order
+// (deterministic).  256 threads, 4 entries each.  Does nothing if run_if is
+// given and run_if[0] == 0.  (A template only so that every translation
 // unit that includes this header gets its own copy.)
 template <int UNUSED = 0>
 __global__ __launch_bounds__(256) void tsqr_gram_partial(int leaves, const double* __restrict__ side,
-                                                         double* __restrict__ partial) {
+                                                         double* __restrict__ partial,
+                                                         const int* __restrict__ run_if) {
+    if (run_if && !run_if[0]) return;
     const int per = (leaves + gridDim.x - 1) / gridDim.x;
     const int b0 = blockIdx.x * per;
     const int b1 = b0 + per < leaves ? b0 + per : leaves;
@@ -489,41 +492,45 @@ __global__ __launch_bounds__(256) void tsqr_gram_partial(int leaves, const doubl
     for (int q = 0; q < 4; ++q) partial[(int64_t)blockIdx.x * 1024 + threadIdx.x + 256 * q] = acc[q];
 }
 
+// L.G = sum of the nparts 32 x 32 FP64 partials, added in order (256
+// threads, thread t owns entries t + 256 e; four chains per entry so that
+// the loads overlap).  The caller synchronises before reading L.G.
+__device__ __forceinline__ void sum_partials(GramLds& L, int nparts, const double* __restrict__ partial, int t) {
+    double s[4][4] = {};
+    int p = 0;
+#pragma unroll 1
+    for (; p + 4 <= nparts; p += 4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) s[e][q] += partial[(int64_t)(p + q) * 1024 + t + 256 * e];
+    }
+    for (; p < nparts; ++p)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s[e][0] += partial[(int64_t)p * 1024 + t + 256 * e];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int idx = t + 256 * e;
+        L.G[idx >> 5][idx & 31] = (s[e][0] + s[e][1]) + (s[e][2] + s[e][3]);
+    }
+}
+
 // Flat Gram path, stage 2 (one workgroup of 256 threads): G = sum of the
 // partials in order, then wave 0 factors it like a superleaf.  If the
 // Cholesky succeeds and ||R||_F ||R^{-1}||_F <= gate for the whole matrix,
-// R (rounded to T) goes to the caller, R^{-1} to side block T0 and ctl[0] = 1;
-// the per-superleaf kernels and the tree then do nothing and tsqr_gram_q
-// forms Q = A R^{-1}.  Otherwise ctl[0] stays 0 and the tree path runs.
+// R (rounded to T) goes to the caller, R^{-1} to side block T0, R itself in
+// FP64 to rkeep (if not null, row-major) and ctl[0] = 1; the per-superleaf
+// kernels and the tree then do nothing and the Q pass forms Q = A R^{-1}.
+// Otherwise ctl[0] stays 0 and the tree path runs.
 template <typename T>
 __global__ __launch_bounds__(256) void tsqr_gram_flat(const TreeDesc d, int nparts,
                                                         const double* __restrict__ partial, T* __restrict__ R,
                                                         int ldr, double* __restrict__ side, int* __restrict__ ctl,
-                                                        double gate) {
+                                                        double gate, double* __restrict__ rkeep) {
     __shared__ GramLds L;
     const int t = threadIdx.x;
     const int n = d.n;
-    {
-        // thread t owns entries t + 256 e; four chains per entry so that the
-        // loads overlap; a fixed order, so the result is deterministic
-        double s[4][4] = {};
-        int p = 0;
-#pragma unroll 1
-        for (; p + 4 <= nparts; p += 4) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) s[e][q] += partial[(int64_t)(p + q) * 1024 + t + 256 * e];
-        }
-        for (; p < nparts; ++p)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) s[e][0] += partial[(int64_t)p * 1024 + t + 256 * e];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int idx = t + 256 * e;
-            L.G[idx >> 5][idx & 31] = (s[e][0] + s[e][1]) + (s[e][2] + s[e][3]);
-        }
-    }
+    sum_partials(L, nparts, partial, t);
     __syncthreads();
     if (t >= 64) return;
     const int lane = t;
@@ -553,6 +560,10 @@ __global__ __launch_bounds__(256) void tsqr_gram_flat(const TreeDesc d, int npar
     if (c < n) {
 #pragma unroll 4
         for (int r = 0; r < n; ++r) R[r + (int64_t)c * ldr] = (r <= c) ? (T)L.G[r][c] : T(0);
+    }
+    if (rkeep) {
+#pragma unroll 4
+        for (int r = 0; r < 32; ++r) rkeep[r * 32 + c] = L.G[r][c];
     }
     double* X = side + (int64_t)d.lv[0].tiles * 1024;
 #pragma unroll

@@ -92,12 +92,16 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d) {
 //   tickets  one int per tile of levels >= 1
 //   flags    one int per level-0 superleaf: 0 Gram path, 1 Householder path
 //   list     count, then the indices of the Householder-path superleaves
-//   ctl      ctl[0] = 1 when the whole matrix took the flat Gram path
-//   partial  min(superleaves, kGramPartials) x 32 x 32 FP64 partial sums of
-//            the superleaf Grams
-//   side     32 x 32 FP64 per superleaf (the Gram matrix, then R^{-1}), and
-//            one more for the whole matrix's R^{-1} on the flat path
+//   ctl      control words: ctl[0] = 1 when the whole matrix took the flat
+//            Gram path; ctl[1], ctl[2] for the mixed path (mixed.hpp)
+//   partial  min(superleaves, kGramPartials) blocks of 32 x 32 FP64 partial
+//            sums of the superleaf Grams, plus kGramPartials2 for a second
+//            stage when there are more than kGramPartials2 of them
+//   side     32 x 32 FP64 per superleaf (the Gram matrix, then R^{-1}), then
+//            three more: the whole matrix's R^{-1} on the flat path (T0),
+//            its R (T0+1) and the mixed path's residuals (T0+2)
 constexpr int kGramPartials = 128;
+constexpr int kGramPartials2 = 8;
 
 struct WsLayout {
     size_t blocks, tickets, flags, list, ctl, partial, side, total;
@@ -113,8 +117,9 @@ inline WsLayout ws_layout(const TreeDesc& d, size_t elem) {
     w.ctl = w.list + align256((leaves + 1) * sizeof(int));
     w.partial = w.ctl + 256;
     const size_t parts = leaves < (size_t)kGramPartials ? leaves : (size_t)kGramPartials;
-    w.side = w.partial + parts * 1024 * sizeof(double);
-    w.total = w.side + align256((leaves + 1) * 1024 * sizeof(double));
+    const size_t parts2 = parts > (size_t)kGramPartials2 ? (size_t)kGramPartials2 : 0;
+    w.side = w.partial + (parts + parts2) * 1024 * sizeof(double);
+    w.total = w.side + align256((leaves + 3) * 1024 * sizeof(double));
     return w;
 }
 

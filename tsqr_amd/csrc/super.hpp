@@ -89,8 +89,10 @@ template <typename T, int N, int RPL, bool VEC>
 __global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_r(const TreeDesc d, const T* __restrict__ A,
                                                                          int lda, T* __restrict__ R, int ldr,
                                                                          T* __restrict__ blocks,
-                                                                         const int* __restrict__ list) {
+                                                                         const int* __restrict__ list,
+                                                                         const int* __restrict__ ctl) {
     __shared__ SuperLds<T, N, RPL> S;
+    if (ctl[0]) return;  // the flat Gram path took the whole matrix
     const int count = list[0];
 #pragma unroll 1
     for (int it = blockIdx.x; it < count; it += gridDim.x) {
@@ -146,9 +148,11 @@ __device__ __forceinline__ void super_q_leaf(const TreeDesc& d, T* __restrict__ 
 template <typename T, int N, int RPL, bool VEC>
 __global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_q(const TreeDesc d, T* __restrict__ A, int lda,
                                                                          const T* __restrict__ blocks,
-                                                                         const int* __restrict__ list) {
+                                                                         const int* __restrict__ list,
+                                                                         const int* __restrict__ ctl) {
     __shared__ SuperLds<T, N, RPL> S;
     __shared__ T stash[N * 64 * RPL];  // wave 0's Q_tile while it works on the stack
+    if (ctl[0]) return;  // the flat Gram path took the whole matrix
     const int count = list[0];
 #pragma unroll 1
     for (int it = blockIdx.x; it < count; it += gridDim.x) {

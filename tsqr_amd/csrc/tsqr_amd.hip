@@ -98,9 +98,12 @@ size_t tsqr_ws_bytes(int m, int n) {
 // Leaf path per dtype (tsqr_set_leaf_path).  FP32 defaults to the Gram path
 // with the Householder fallback; FP64 has no wider type to accumulate the
 // Gram matrix in, so it defaults to Householder leaves.
+// The mixed entry points have their own gate and residual tolerance
+// (tsqr_set_mixed).
 std::mutex g_opts_mu;
-LaunchOpts g_opts_f32{kLeafAuto, 1024.0};
-LaunchOpts g_opts_f64{kLeafHouseholder, 1024.0};
+LaunchOpts g_opts_f32{kLeafAuto, 1024.0, 0.0};
+LaunchOpts g_opts_f64{kLeafHouseholder, 1024.0, 0.0};
+LaunchOpts g_opts_mixed{kLeafMixed, 1024.0, 8e-7};
 
 template <typename T>
 LaunchOpts current_opts() {
@@ -108,23 +111,29 @@ LaunchOpts current_opts() {
     return sizeof(T) == 4 ? g_opts_f32 : g_opts_f64;
 }
 
+LaunchOpts mixed_opts() {
+    std::lock_guard<std::mutex> lock(g_opts_mu);
+    return g_opts_mixed;
+}
+
 // Runs one TSQR; work must hold tsqr_ws_bytes<T>(m, n) bytes.
 template <typename T>
-int tsqr_core(int m, int n, T* A, int lda, T* R, int ldr, void* work, hipStream_t st) {
+int tsqr_core(int m, int n, T* A, int lda, T* R, int ldr, void* work, hipStream_t st, bool mixed = false) {
     TreeDesc d;
     if (!plan_tsqr<T>(m, n, &d)) return TSQR_ERR_SHAPE;
-    return tsqr_launch<T>(d, A, lda, R, ldr, work, current_opts<T>(), st);
+    return tsqr_launch<T>(d, A, lda, R, ldr, work, mixed ? mixed_opts() : current_opts<T>(), st);
 }
 
 template <typename T>
-int tsqr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t work_bytes, hipStream_t st) {
+int tsqr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t work_bytes, hipStream_t st,
+                bool mixed = false) {
     int e = check_args(m, n, A, lda, R, ldr);
     if (e) return e;
     if (n > 32) return TSQR_ERR_SHAPE;
     const size_t need = tsqr_ws_bytes<T>(m, n);
     if (need == 0) return TSQR_ERR_SHAPE;
     if (work_bytes < need || (need > 0 && !work)) return TSQR_ERR_WORKSPACE;
-    return tsqr_core<T>(m, n, A, lda, R, ldr, work, st);
+    return tsqr_core<T>(m, n, A, lda, R, ldr, work, st, mixed);
 }
 
 // ------------------------------------------------------------------------
@@ -292,9 +301,9 @@ int tsqr_set_leaf_path(int dtype, int path, double gate) {
         return TSQR_ERR_ARG;
     std::lock_guard<std::mutex> lock(g_opts_mu);
     if (dtype == TSQR_FLOAT)
-        g_opts_f32 = LaunchOpts{path, gate};
+        g_opts_f32 = LaunchOpts{path, gate, 0.0};
     else if (dtype == TSQR_DOUBLE)
-        g_opts_f64 = LaunchOpts{path, gate};
+        g_opts_f64 = LaunchOpts{path, gate, 0.0};
     else
         return TSQR_ERR_ARG;
     return TSQR_OK;
@@ -328,6 +337,42 @@ int tsqr_leaf_stats(const void* work, int m, int n, int dtype, int* gram_leaves,
     return TSQR_OK;
 }
 
+int tsqr_set_mixed(double gate, double tol) {
+    if (!(gate >= 1.0) || !(tol > 0.0)) return TSQR_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_opts_mu);
+    g_opts_mixed.gate = gate;
+    g_opts_mixed.tol = tol;
+    return TSQR_OK;
+}
+
+int tsqr_mixed_stats(const void* work, int m, int n, int* flat, int* passes, int* check, double* residual) {
+    if (!work || !flat || !passes || !check || !residual) return TSQR_ERR_ARG;
+    if (n < 1 || m < n || n > 32) return TSQR_ERR_SHAPE;
+    TreeDesc d;
+    if (!plan_tsqr<float>(m, n, &d)) return TSQR_ERR_SHAPE;
+    *flat = 0;
+    *passes = 0;
+    *check = 0;
+    *residual = 0.0;
+    if (d.lv[0].rows <= d.H) return TSQR_OK;  // single tile: Householder
+    const WsLayout w = ws_layout(d, sizeof(float));
+    int ctl[3];
+    double e[2];
+    const char* base = reinterpret_cast<const char*>(work);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(ctl, base + w.ctl, sizeof(ctl), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(e, base + w.side + (size_t)(d.lv[0].tiles + 2) * 1024 * sizeof(double), sizeof(e),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+        return TSQR_ERR_HIP;
+    *flat = ctl[0];
+    if (ctl[0]) {
+        *passes = ctl[1] ? 2 : 1;
+        *check = ctl[2];
+        *residual = ctl[1] ? e[1] : e[0];
+    }
+    return TSQR_OK;
+}
+
 size_t tsqr_workspace_bytes(int m, int n, int dtype) {
     if (dtype == TSQR_FLOAT) return tsqr_ws_bytes<float>(m, n);
     if (dtype == TSQR_DOUBLE) return tsqr_ws_bytes<double>(m, n);
@@ -347,6 +392,10 @@ int tsqr_d_stream(int m, int n, double* A, int lda, double* R, int ldr, void* wo
                   void* stream) {
     return tsqr_stream<double>(m, n, A, lda, R, ldr, work, work_bytes, (hipStream_t)stream);
 }
+int tsqr_mixed_s_stream(int m, int n, float* A, int lda, float* R, int ldr, void* work, size_t work_bytes,
+                        void* stream) {
+    return tsqr_stream<float>(m, n, A, lda, R, ldr, work, work_bytes, (hipStream_t)stream, true);
+}
 int qr_s_stream(int m, int n, float* A, int lda, float* R, int ldr, void* work, size_t work_bytes, void* stream) {
     return qr_stream<float>(m, n, A, lda, R, ldr, work, work_bytes, (hipStream_t)stream);
 }
@@ -364,6 +413,17 @@ int tsqr_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int 
     e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
     if (e) return e;
     return tsqr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+}
+int tsqr_mixed_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int ldwork) {
+    int e = check_args(m, n, A, lda, R, ldr);
+    if (e) return e;
+    const size_t need = tsqr_ws_bytes<float>(m, n);
+    if (need == 0) return TSQR_ERR_SHAPE;
+    void* ws = nullptr;
+    size_t bytes = 0;
+    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
+    if (e) return e;
+    return tsqr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr, true);
 }
 int tsqr_d(int m, int n, double* A, int lda, double* R, int ldr, double* work, int ldwork) {
     int e = check_args(m, n, A, lda, R, ldr);

@@ -55,11 +55,14 @@ enum LeafPath {
     kLeafAuto = 0,         // flat Gram path if the whole matrix passes the gate, else kLeafGramTree
     kLeafHouseholder = 1,  // Householder for every superleaf
     kLeafGramTree = 2,     // Gram path per superleaf (Householder where rejected) + Householder tree
+    kLeafMixed = 3,        // FP32 only: BF16x3 CholeskyQR with residual check and optional
+                           // second pass (mixed.hpp), Householder tree if the gate rejects
 };
 
 struct LaunchOpts {
     int path;     // LeafPath
     double gate;  // largest accepted ||R||_F ||R^{-1}||_F on the Gram path
+    double tol;   // mixed path: largest accepted ||Q^T Q - I||_F
 };
 
 // Optional per-phase timing (tsqr_timing_* in the C ABI).  When enabled,

@@ -1,7 +1,7 @@
 // tsqr_launch_impl.hpp -- body of tsqr_launch<T>, included once per dtype.
 //
 // Launch sequence for one call (m > H):
-//   init          tickets = 0, superleaf flags = path
+//   init          tickets = 0, superleaf flags = path, control words = 0
 //   gram_g        (auto path) FP64 Gram matrix of every superleaf
 //   gram_partial, gram_flat
 //                 (auto path) sum of all superleaf Grams and its Cholesky;
@@ -15,9 +15,14 @@
 //   down          levels root-1 .. 1, one launch each
 //   gram_q        M = R^{-1} S, then Q = A M, for the Gram superleaves
 //   super_q       Householder Q for the flagged superleaves
+// The mixed path (FP32) replaces the Gram kernels by mixed_gram, the flat
+// reduction, mixed_q + mixed_check (pass 1) and mixed_q + mixed_check
+// (pass 2, a no-op unless the check asks for it); every superleaf is flagged
+// Householder so that the tree path takes over if the flat gate rejects.
 // A single tile (m <= H) is one tsqr_single launch.
 #pragma once
 #include "../../include/tsqr_amd.h"
+#include "mixed.hpp"
 #include "super.hpp"
 #include "tsqr_launch.hpp"
 
@@ -29,7 +34,7 @@ inline bool aligned16(const void* p, int ld, size_t elem) {
 
 namespace {
 // tickets = 0; flags = value; the Householder list is empty (value 0) or
-// holds every superleaf (value 1); the flat-path flag is cleared.
+// holds every superleaf (value 1); the control words are cleared.
 __global__ void init_flags(int* __restrict__ tickets, int nt, int* __restrict__ flags, int* __restrict__ list,
                            int nf, int value, int* __restrict__ ctl) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -38,10 +43,25 @@ __global__ void init_flags(int* __restrict__ tickets, int nt, int* __restrict__ 
         flags[i] = value;
         if (value) list[1 + i] = i;
     }
-    if (i == 0) {
-        list[0] = value ? nf : 0;
-        ctl[0] = 0;
+    if (i < 64) ctl[i] = 0;
+    if (i == 0) list[0] = value ? nf : 0;
+}
+
+// Sums `count` 32 x 32 FP64 blocks at src into at most kGramPartials2
+// partials (two deterministic stages); returns the partials and their count.
+inline const double* reduce_grams(int count, const double* src, double* partial, const int* run_if, int* nparts,
+                                  hipStream_t st) {
+    const int p1 = count < kGramPartials ? count : kGramPartials;
+    hipLaunchKernelGGL(tsqr_gram_partial<>, dim3(p1), dim3(256), 0, st, count, src, partial, run_if);
+    if (p1 <= kGramPartials2) {
+        *nparts = p1;
+        return partial;
     }
+    double* second = partial + (size_t)p1 * 1024;
+    hipLaunchKernelGGL(tsqr_gram_partial<>, dim3(kGramPartials2), dim3(256), 0, st, p1, (const double*)partial,
+                       second, run_if);
+    *nparts = kGramPartials2;
+    return second;
 }
 }  // namespace
 
@@ -66,30 +86,44 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     double* partial = reinterpret_cast<double*>(base + w.partial);
     double* side = reinterpret_cast<double*>(base + w.side);
     const int leaves = d.lv[0].tiles;
-    const bool gram = o.path != kLeafHouseholder;
+    const bool mixed = o.path == kLeafMixed;
+    const bool gram = o.path == kLeafAuto || o.path == kLeafGramTree;
     const int force = gram ? 0 : 1;
     constexpr int NT = N > 16 ? 2 : 1;
+    if constexpr (sizeof(T) != 4) {
+        if (mixed) return TSQR_ERR_ARG;
+    }
 
     const int ninit = d.total_ctr > leaves ? d.total_ctr : leaves;
-    hipLaunchKernelGGL(init_flags, dim3((ninit + 255) / 256), dim3(256), 0, st, tickets, d.total_ctr, flags, list,
-                       leaves, force, ctl);
+    hipLaunchKernelGGL(init_flags, dim3((ninit + 63 + 255) / 256), dim3(256), 0, st, tickets, d.total_ctr, flags,
+                       list, leaves, force, ctl);
     // Householder superleaf kernels: a grid-stride loop over the list
     const int super_grid = leaves < 1024 ? leaves : 1024;
+    double* rkeep = side + (size_t)(leaves + 1) * 1024;
     if (timed) timing_mark(kPhaseLeafR, true, st);
+    if constexpr (sizeof(T) == 4) {
+        if (mixed) {
+            hipLaunchKernelGGL((tsqr_mixed_gram<NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const float*)A, lda,
+                               side);
+            int np = 0;
+            const double* parts = reduce_grams(leaves, side, partial, nullptr, &np, st);
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, np, parts, R, ldr, side, ctl,
+                               o.gate, rkeep);
+        }
+    }
     if (gram) {
         hipLaunchKernelGGL((tsqr_gram_g<T, NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const T*)A, lda, side);
         if (leaves >= 2 && o.path == kLeafAuto) {
-            const int parts = leaves < kGramPartials ? leaves : kGramPartials;
-            hipLaunchKernelGGL(tsqr_gram_partial<>, dim3(parts), dim3(256), 0, st, leaves, (const double*)side,
-                               partial);
-            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, parts, (const double*)partial, R,
-                               ldr, side, ctl, o.gate);
+            int np = 0;
+            const double* parts = reduce_grams(leaves, side, partial, nullptr, &np, st);
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, np, parts, R, ldr, side, ctl,
+                               o.gate, (double*)nullptr);
         }
         hipLaunchKernelGGL((tsqr_gram_chol<T>), dim3((leaves + 1) / 2), dim3(64), 0, st, d, R, ldr, blocks, side,
                            flags, list, (const int*)ctl, o.gate);
     }
     hipLaunchKernelGGL((tsqr_super_r<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st, d,
-                       (const T*)A, lda, R, ldr, blocks, (const int*)list);
+                       (const T*)A, lda, R, ldr, blocks, (const int*)list, (const int*)ctl);
     if (timed) timing_mark(kPhaseLeafR, false, st);
     if (d.levels >= 2) {
         if (timed) timing_mark(kPhaseTree, true, st);
@@ -103,11 +137,23 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
                            (const int*)ctl);
     if (timed) timing_mark(kPhaseDown, false, st);
     if (timed) timing_mark(kPhaseLeafQ, true, st);
+    if constexpr (sizeof(T) == 4) {
+        if (mixed) {
+            for (int pass = 1; pass <= 2; ++pass) {
+                hipLaunchKernelGGL((tsqr_mixed_q<NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (float*)A, lda, side,
+                                   (const int*)ctl, pass);
+                int np = 0;
+                const double* parts = reduce_grams(leaves, side, partial, ctl + (pass == 1 ? 0 : 1), &np, st);
+                hipLaunchKernelGGL(tsqr_mixed_check<>, dim3(1), dim3(256), 0, st, d, np, parts, (float*)R, ldr, side,
+                                   ctl, pass, o.tol);
+            }
+        }
+    }
     if (gram)
         hipLaunchKernelGGL((tsqr_gram_q<T, VEC>), dim3(leaves), dim3(64), 0, st, d, A, lda, (const double*)side,
                            (const T*)blocks, (const int*)flags, (const int*)ctl);
     hipLaunchKernelGGL((tsqr_super_q<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st, d, A,
-                       lda, (const T*)blocks, (const int*)list);
+                       lda, (const T*)blocks, (const int*)list, (const int*)ctl);
     if (timed) timing_mark(kPhaseLeafQ, false, st);
     return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
 }
