@@ -120,6 +120,45 @@ int tsqr_set_mixed(double gate, double tol);
  * ||Q^T Q - I||_F estimate.  Synchronises the device.  Diagnostic. */
 int tsqr_mixed_stats(const void *work, int m, int n, int *flat, int *passes, int *check, double *residual);
 
+/* Batched TSQR of many independent matrices of uneven shape (no counterpart
+ * in the reference, which factors one matrix per call).  Matrix i is m[i] x
+ * n[i] (1 <= n[i] <= 64, m[i] >= n[i]) column-major at the device pointer
+ * A[i] (leading dimension lda[i]) and is overwritten by its explicit Q; its
+ * n[i] x n[i] R goes to R[i] (ldr[i]).  n[i] <= 32 is one TSQR; for
+ * 32 < n[i] <= 64 the columns are split at 32 and coupled by one block
+ * Gram-Schmidt step (R12 = Q1^T A2, A2 -= Q1 R12, FP64 MFMA products), as
+ * the recursive qr does (block_QR.h:72-95).
+ * method TSQR_BATCH_PLAIN: every panel is the Householder TSQR (the
+ *        reference's method), in the working precision (float or double);
+ * method TSQR_BATCH_MIXED (float only): every panel is the mixed-precision
+ *        TSQR of tsqr_mixed_s (BF16x3 CholeskyQR, residual check, optional
+ *        second pass, Householder if the gate rejects) and the coupling step
+ *        is applied twice (BCGS2).
+ * tsqr_batch_create validates the shapes, plans the trees and the device
+ * work lists of (matrix, row block) items once and copies them to device
+ * memory it owns; the pointers are fixed for the life of the plan.
+ * tsqr_batch_run enqueues the whole batch on `stream` (a few launches, one
+ * per phase, each over every matrix; no host synchronisation, no memset),
+ * tsqr_batch_graph_launch captures the same launches into a hipGraph on
+ * its first call for a given workspace and replays the graph on `stream`.
+ * Calls on one plan must not overlap.  work: tsqr_batch_workspace_bytes. */
+#define TSQR_BATCH_PLAIN 0
+#define TSQR_BATCH_MIXED 1
+typedef struct tsqr_batch tsqr_batch;
+int tsqr_batch_create(int dtype, int method, int count, const int *m, const int *n, void *const *A, const int *lda,
+                      void *const *R, const int *ldr, tsqr_batch **plan);
+size_t tsqr_batch_workspace_bytes(const tsqr_batch *plan);
+int tsqr_batch_run(tsqr_batch *plan, void *work, size_t work_bytes, void *stream);
+int tsqr_batch_graph_launch(tsqr_batch *plan, void *work, size_t work_bytes, void *stream);
+/* Gate and residual tolerance of a mixed plan (defaults 1024 and 8e-7). */
+int tsqr_batch_set_mixed(tsqr_batch *plan, double gate, double tol);
+/* After a run (synchronises the device): panels that took the CholeskyQR
+ * path, panels that ran Householder, panels that needed the second pass and
+ * panels whose final residual check failed. */
+int tsqr_batch_stats(const tsqr_batch *plan, const void *work, int *cholesky, int *householder, int *second_pass,
+                     int *check_failed);
+int tsqr_batch_destroy(tsqr_batch *plan);
+
 /* Chooses how level-0 superleaves (up to 8 Householder tiles of rows each)
  * are factored for one dtype, process-wide.  No counterpart in the
  * reference, whose tiles are always Householder (kernelQR.h:69-291).

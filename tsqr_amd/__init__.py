@@ -11,6 +11,9 @@ that takes torch tensors only as holders of device memory and streams:
                                  float32 TSQR on the BF16 matrix cores
                                  (CholeskyQR with a residual check and an
                                  optional second pass), same convention
+    Batch(mats, method="plain"|"mixed")
+                                 many uneven TSQRs (n <= 64) planned once,
+                                 run as a few launches or one hipGraph
 
 A column-major m x n tensor is one whose strides are (1, lda), for example
 ``colmajor(torch.randn(m, n))``.  Errors from the library raise TsqrError
@@ -28,9 +31,10 @@ from ._lib import (  # noqa: F401
     qr_workspace_bytes,
     tsqr_workspace_bytes,
 )
-from .api import colmajor, empty_colmajor, qr, tsqr, tsqr_mixed  # noqa: F401
+from .api import Batch, colmajor, empty_colmajor, qr, tsqr, tsqr_mixed  # noqa: F401
 
 __all__ = [
+    "Batch",
     "LIB_PATH",
     "TsqrError",
     "colmajor",

@@ -13,6 +13,8 @@ TSQR_DOUBLE = 2
 TSQR_LEAF_AUTO = 0
 TSQR_LEAF_HOUSEHOLDER = 1
 TSQR_LEAF_GRAM_TREE = 2
+TSQR_BATCH_PLAIN = 0
+TSQR_BATCH_MIXED = 1
 
 # Every symbol include/tsqr_amd.h declares.
 EXPORTS = (
@@ -35,6 +37,13 @@ EXPORTS = (
     "tsqr_mixed_s_stream",
     "tsqr_set_mixed",
     "tsqr_mixed_stats",
+    "tsqr_batch_create",
+    "tsqr_batch_workspace_bytes",
+    "tsqr_batch_run",
+    "tsqr_batch_graph_launch",
+    "tsqr_batch_set_mixed",
+    "tsqr_batch_stats",
+    "tsqr_batch_destroy",
 )
 
 
@@ -81,6 +90,22 @@ def lib() -> ctypes.CDLL:
     pi = ctypes.POINTER(i32)
     L.tsqr_mixed_stats.argtypes = [vp, i32, i32, pi, pi, pi, ctypes.POINTER(ctypes.c_double)]
     L.tsqr_mixed_stats.restype = i32
+    ip = ctypes.POINTER(i32)
+    L.tsqr_batch_create.argtypes = [i32, i32, i32, ip, ip, ctypes.POINTER(vp), ip, ctypes.POINTER(vp), ip,
+                                    ctypes.POINTER(vp)]
+    L.tsqr_batch_create.restype = i32
+    L.tsqr_batch_workspace_bytes.argtypes = [vp]
+    L.tsqr_batch_workspace_bytes.restype = sz
+    for name in ("tsqr_batch_run", "tsqr_batch_graph_launch"):
+        f = getattr(L, name)
+        f.argtypes = [vp, vp, sz, vp]
+        f.restype = i32
+    L.tsqr_batch_set_mixed.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    L.tsqr_batch_set_mixed.restype = i32
+    L.tsqr_batch_stats.argtypes = [vp, vp, ip, ip, ip, ip]
+    L.tsqr_batch_stats.restype = i32
+    L.tsqr_batch_destroy.argtypes = [vp]
+    L.tsqr_batch_destroy.restype = i32
     L.tsqr_timing_enable.argtypes = [i32]
     L.tsqr_timing_enable.restype = i32
     L.tsqr_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), i32]

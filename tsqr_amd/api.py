@@ -70,3 +70,80 @@ def tsqr_mixed(A: torch.Tensor, R: torch.Tensor | None = None, work: torch.Tenso
     BF16x3 CholeskyQR with an FP32 residual check and an optional second pass.
     Same conventions as tsqr."""
     return _run("tsqr_mixed", A, R, work, stream)
+
+
+class Batch:
+    """A planned batch of independent TSQRs (tsqr_batch_* in the C ABI).
+
+    mats: column-major m_i x n_i GPU tensors (n_i <= 64), all float32 or all
+    float64; each is overwritten by its Q when the batch runs.  Rs: the
+    n_i x n_i R tensors (allocated if None).  method: "plain" (Householder)
+    or "mixed" (float32 only).  The plan holds the tensors' pointers, so the
+    tensors must stay alive and in place while the plan is used."""
+
+    def __init__(self, mats, Rs=None, method: str = "plain"):
+        import ctypes
+
+        from ._lib import TSQR_BATCH_MIXED, TSQR_BATCH_PLAIN
+
+        if not mats:
+            raise ValueError("empty batch")
+        dtype = mats[0].dtype
+        code = _code(dtype)
+        self.mats = list(mats)
+        self.Rs = list(Rs) if Rs is not None else [empty_colmajor(a.shape[1], a.shape[1], dtype, a.device)
+                                                     for a in self.mats]
+        k = len(self.mats)
+        i32 = ctypes.c_int * k
+        vp = ctypes.c_void_p * k
+        m = i32(*[a.shape[0] for a in self.mats])
+        n = i32(*[a.shape[1] for a in self.mats])
+        lda = i32(*[_ld(a, "A") for a in self.mats])
+        ldr = i32(*[_ld(r, "R") for r in self.Rs])
+        A = vp(*[a.data_ptr() for a in self.mats])
+        R = vp(*[r.data_ptr() for r in self.Rs])
+        meth = {"plain": TSQR_BATCH_PLAIN, "mixed": TSQR_BATCH_MIXED}[method]
+        self._h = ctypes.c_void_p()
+        st = lib().tsqr_batch_create(code, meth, k, m, n, A, lda, R, ldr, ctypes.byref(self._h))
+        if st != 0:
+            raise TsqrError(st, f"tsqr_batch_create(count={k}, method={method})")
+        nbytes = int(lib().tsqr_batch_workspace_bytes(self._h))
+        self.work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.mats[0].device)
+
+    def _stream(self, stream):
+        return (stream if stream is not None else torch.cuda.current_stream(self.work.device)).cuda_stream
+
+    def run(self, stream=None):
+        st = lib().tsqr_batch_run(self._h, self.work.data_ptr(), self.work.numel(), self._stream(stream))
+        if st != 0:
+            raise TsqrError(st, "tsqr_batch_run")
+
+    def graph_launch(self, stream=None):
+        st = lib().tsqr_batch_graph_launch(self._h, self.work.data_ptr(), self.work.numel(), self._stream(stream))
+        if st != 0:
+            raise TsqrError(st, "tsqr_batch_graph_launch")
+
+    def set_mixed(self, gate: float = 1024.0, tol: float = 8e-7):
+        st = lib().tsqr_batch_set_mixed(self._h, gate, tol)
+        if st != 0:
+            raise TsqrError(st, "tsqr_batch_set_mixed")
+
+    def stats(self) -> dict:
+        import ctypes
+
+        v = [ctypes.c_int(0) for _ in range(4)]
+        st = lib().tsqr_batch_stats(self._h, self.work.data_ptr(), *[ctypes.byref(x) for x in v])
+        if st != 0:
+            raise TsqrError(st, "tsqr_batch_stats")
+        return dict(zip(("cholesky", "householder", "second_pass", "check_failed"), (x.value for x in v)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().tsqr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

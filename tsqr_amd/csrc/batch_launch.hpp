@@ -1,0 +1,46 @@
+// batch_launch.hpp -- host-side description of a planned batch (device
+// pointers into the plan's memory) and the launch sequence, instantiated per
+// dtype in tsqr_f32.hip / tsqr_f64.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "plan.hpp"
+
+namespace tsqr {
+
+struct BatchProb;
+struct BatchCouple;
+
+struct ItemList {
+    const int2* items = nullptr;  // device
+    int count = 0;
+};
+
+// One stage of panel problems (all problems reference the batch-wide
+// problem array by index).
+struct BatchStage {
+    ItemList probs;              // (problem, 0) for every problem of the stage
+    ItemList leaves;             // every superleaf
+    ItemList level1;             // every level-1 tree block
+    std::vector<ItemList> down;  // down[l]: every tile of level l (l >= 1)
+};
+
+struct BatchDev {
+    int method = 0;  // 0 Householder, 1 mixed (FP32)
+    double gate = 1024.0, tol = 8e-7;
+    const BatchProb* probs = nullptr;  // device
+    int nprobs = 0;
+    const BatchCouple* couples = nullptr;  // device
+    int ncouples = 0;
+    ItemList couple_rows;
+    BatchStage stage[2];  // first panels (and whole matrices with n <= 32), second panels
+};
+
+template <typename T>
+int batch_launch(const BatchDev& b, void* work, hipStream_t st);
+extern template int batch_launch<float>(const BatchDev&, void*, hipStream_t);
+extern template int batch_launch<double>(const BatchDev&, void*, hipStream_t);
+
+}  // namespace tsqr

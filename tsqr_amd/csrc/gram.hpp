@@ -515,31 +515,21 @@ __device__ __forceinline__ void sum_partials(GramLds& L, int nparts, const doubl
     }
 }
 
-// Flat Gram path, stage 2 (one workgroup of 256 threads): G = sum of the
-// partials in order, then wave 0 factors it like a superleaf.  If the
-// Cholesky succeeds and ||R||_F ||R^{-1}||_F <= gate for the whole matrix,
-// R (rounded to T) goes to the caller, R^{-1} to side block T0, R itself in
-// FP64 to rkeep (if not null, row-major) and ctl[0] = 1; the per-superleaf
-// kernels and the tree then do nothing and the Q pass forms Q = A R^{-1}.
-// Otherwise ctl[0] stays 0 and the tree path runs.
+// Cholesky of the whole-matrix Gram matrix in L.G by one wave (lanes
+// 0..63; both half-waves factor the same matrix).  If it succeeds and
+// ||R||_F ||R^{-1}||_F <= gate: R (rounded to T) goes to the caller, R^{-1}
+// (FP64, row-major) to X, R itself in FP64 to rkeep (if not null) and
+// ctl[0] = 1.  Otherwise nothing is written.
 template <typename T>
-__global__ __launch_bounds__(256) void tsqr_gram_flat(const TreeDesc d, int nparts,
-                                                        const double* __restrict__ partial, T* __restrict__ R,
-                                                        int ldr, double* __restrict__ side, int* __restrict__ ctl,
-                                                        double gate, double* __restrict__ rkeep) {
-    __shared__ GramLds L;
-    const int t = threadIdx.x;
-    const int n = d.n;
-    sum_partials(L, nparts, partial, t);
-    __syncthreads();
-    if (t >= 64) return;
-    const int lane = t;
+__device__ __forceinline__ void flat_factor(GramLds& L, int n, T* __restrict__ R, int ldr, double* __restrict__ X,
+                                            int* __restrict__ ctl, double gate, double* __restrict__ rkeep,
+                                            int lane) {
     const int c = lane & 31;
     double nr;
     bool ok;
     {
         double g[32];
-        ok = chol_wave(L, n, lane, g);  // both half-waves factor the same G
+        ok = chol_wave(L, n, lane, g);
         nr = 0.0;
 #pragma unroll
         for (int r = 0; r < 32; ++r) nr = fmat(g[r], g[r], nr);
@@ -565,10 +555,26 @@ __global__ __launch_bounds__(256) void tsqr_gram_flat(const TreeDesc d, int npar
 #pragma unroll 4
         for (int r = 0; r < 32; ++r) rkeep[r * 32 + c] = L.G[r][c];
     }
-    double* X = side + (int64_t)d.lv[0].tiles * 1024;
 #pragma unroll
     for (int r = 0; r < 32; ++r) X[r * 32 + c] = x[r];
     if (c == 0) ctl[0] = 1;
+}
+
+// Flat Gram path, stage 2 (one workgroup of 256 threads): G = sum of the
+// partials in order, then flat_factor with X = side block T0.  If the gate
+// passes, the per-superleaf kernels and the tree do nothing and the Q pass
+// forms Q = A R^{-1}; otherwise ctl[0] stays 0 and the tree path runs.
+template <typename T>
+__global__ __launch_bounds__(256) void tsqr_gram_flat(const TreeDesc d, int nparts,
+                                                        const double* __restrict__ partial, T* __restrict__ R,
+                                                        int ldr, double* __restrict__ side, int* __restrict__ ctl,
+                                                        double gate, double* __restrict__ rkeep) {
+    __shared__ GramLds L;
+    const int t = threadIdx.x;
+    sum_partials(L, nparts, partial, t);
+    __syncthreads();
+    if (t >= 64) return;
+    flat_factor<T>(L, d.n, R, ldr, side + (int64_t)d.lv[0].tiles * 1024, ctl, gate, rkeep, t);
 }
 
 }  // namespace tsqr

@@ -147,18 +147,14 @@ __device__ __forceinline__ void load8(float (&x)[8], const float* __restrict__ b
 // four steps of a 128-row batch are loaded together and the next batch is in
 // flight while the current one is multiplied.
 template <int NT, bool VEC>
-__global__ __launch_bounds__(64) void tsqr_mixed_gram(const TreeDesc d, const float* __restrict__ A, int lda,
-                                                      double* __restrict__ side) {
-    const int lane = threadIdx.x;
-    const int n = d.n;
-    const int leaf = blockIdx.x;
-    const LevelDesc& L0 = d.lv[0];
+__device__ __forceinline__ void mixed_gram_body(const LevelDesc& L0, int n, const float* __restrict__ A, int lda,
+                                                double* __restrict__ G, double* sums, int leaf) {
+    const int lane = threadIdx.x & 63;
     const int rows = leaf_rows_of(L0, leaf);
     const float* base = A + (int64_t)leaf * L0.tile_rows;
     const int col = lane & 15;
     const int rq = 8 * (lane >> 4);
     constexpr int D = 4;  // 32-row steps per batch
-    __shared__ double sums[12 * 64];
     GramAcc32 g;
     g.sums = sums;
     g.zero(lane);
@@ -193,7 +189,14 @@ __global__ __launch_bounds__(64) void tsqr_mixed_gram(const TreeDesc d, const fl
                 if constexpr (NT == 2) c1[q][t] = p1[q][t];
             }
     }
-    g.store(side + (int64_t)leaf * 1024, lane);
+    g.store(G, lane);
+}
+
+template <int NT, bool VEC>
+__global__ __launch_bounds__(64) void tsqr_mixed_gram(const TreeDesc d, const float* __restrict__ A, int lda,
+                                                      double* __restrict__ side) {
+    __shared__ double sums[12 * 64];
+    mixed_gram_body<NT, VEC>(d.lv[0], d.n, A, lda, side + (int64_t)blockIdx.x * 1024, sums, blockIdx.x);
 }
 
 // Pass 1b / 2: Q = A X in place for every superleaf (X upper triangular,
@@ -209,15 +212,11 @@ __global__ __launch_bounds__(64) void tsqr_mixed_gram(const TreeDesc d, const fl
 // k = 8(l/16) + s <-> row 16(v - 2h) + 4(l/16) + t: a bijection onto the 32
 // rows, identical for both operands.
 template <int NT, bool VEC>
-__global__ __launch_bounds__(64) void tsqr_mixed_q(const TreeDesc d, float* __restrict__ A, int lda,
-                                                   double* __restrict__ side, const int* __restrict__ ctl, int pass) {
-    if (!ctl[0] || (pass == 2 && !ctl[1])) return;
-    const int leaf = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int n = d.n;
-    const LevelDesc& L0 = d.lv[0];
+__device__ __forceinline__ void mixed_q_body(const LevelDesc& L0, int n, float* __restrict__ A, int lda,
+                                             const double* __restrict__ X, double* __restrict__ Gout, double* sums,
+                                             int leaf) {
+    const int lane = threadIdx.x & 63;
     const int li = lane & 15, lq = lane >> 4;
-    const double* X = side + (int64_t)L0.tiles * 1024;
     // B operand of k-group kg (k = 4 kg + l/16), column tile jt
     double mb[8][2];
 #pragma unroll
@@ -233,7 +232,6 @@ __global__ __launch_bounds__(64) void tsqr_mixed_q(const TreeDesc d, float* __re
         const int k = 4 * kg + lq;
         if (kg < nk) raw[kg] = load_raw4<float, VEC>(base, lda, 4 * li, k < n ? k : n - 1, rows, n);
     }
-    __shared__ double sums[12 * 64];
     GramAcc32 g;
     g.sums = sums;
     g.zero(lane);
@@ -297,28 +295,28 @@ __global__ __launch_bounds__(64) void tsqr_mixed_q(const TreeDesc d, float* __re
         }
         g.flush<NT>(lane);
     }
-    g.store(side + (int64_t)leaf * 1024, lane);
+    g.store(Gout, lane);
 }
 
-// Check after pass `pass` (one workgroup of 256 threads): G = sum of the
-// nparts partials (in order), e = ||G - I||_F over the leading n x n block,
-// stored in side block T0+2.  Pass 1: e <= tol finishes; otherwise
-// R2 = chol(G), R = R2 R1 (R1 in FP64 in side block T0+1) to the caller,
-// X = R2^{-1} to side block T0 and ctl[1] = 1.  Pass 2: ctl[2] = (e > tol).
-// (A template only so that every translation unit gets its own copy.)
-template <int UNUSED = 0>
-__global__ __launch_bounds__(256) void tsqr_mixed_check(const TreeDesc d, int nparts,
-                                                        const double* __restrict__ partial, float* __restrict__ R,
-                                                        int ldr, double* __restrict__ side, int* __restrict__ ctl,
-                                                        int pass, double tol) {
-    __shared__ GramLds L;
+template <int NT, bool VEC>
+__global__ __launch_bounds__(64) void tsqr_mixed_q(const TreeDesc d, float* __restrict__ A, int lda,
+                                                   double* __restrict__ side, const int* __restrict__ ctl, int pass) {
+    __shared__ double sums[12 * 64];
     if (!ctl[0] || (pass == 2 && !ctl[1])) return;
-    const int t = threadIdx.x;
-    const int n = d.n;
-    sum_partials(L, nparts, partial, t);
-    __syncthreads();
-    if (t >= 64) return;
-    const int lane = t;
+    mixed_q_body<NT, VEC>(d.lv[0], d.n, A, lda, side + (int64_t)d.lv[0].tiles * 1024,
+                          side + (int64_t)blockIdx.x * 1024, sums, blockIdx.x);
+}
+
+// Residual check after pass `pass`, by one wave (lanes 0..63), on the Gram
+// matrix of the pass's output in L.G: e = ||G - I||_F over the leading
+// n x n block goes to misc[pass - 1].  Pass 1: e <= tol finishes;
+// otherwise R2 = chol(G), R = R2 R1 (R1 in FP64, row-major) to the caller,
+// X = R2^{-1} (FP64, row-major) and ctl[1] = 1.  Pass 2: ctl[2] = (e > tol).
+// A second Cholesky that breaks down sets ctl[2] = 2 and keeps Q1, R1.
+__device__ __forceinline__ void mixed_check_body(GramLds& L, int n, float* __restrict__ R, int ldr,
+                                                 double* __restrict__ X, const double* __restrict__ R1,
+                                                 double* __restrict__ misc, int* __restrict__ ctl, int pass,
+                                                 double tol, int lane) {
     const int c = lane & 31;
     double e = 0.0;
     if (c < n) {
@@ -329,7 +327,6 @@ __global__ __launch_bounds__(256) void tsqr_mixed_check(const TreeDesc d, int np
         }
     }
     e = __builtin_sqrt(sum_half(e));
-    double* misc = side + (int64_t)(d.lv[0].tiles + 2) * 1024;
     if (pass == 2) {
         if (lane == 0) {
             misc[1] = e;
@@ -350,26 +347,44 @@ __global__ __launch_bounds__(256) void tsqr_mixed_check(const TreeDesc d, int np
         return;
     }
     // column c of R = R2 R1: sum over r <= k <= c of R2[r][k] R1[k][c]
-    const double* R1 = side + (int64_t)(d.lv[0].tiles + 1) * 1024;
-    double r1c[32];
-#pragma unroll
-    for (int k = 0; k < 32; ++k) r1c[k] = R1[k * 32 + c];
+    // (R1 read from memory in the loop: this runs once per matrix and the
+    // kernels it is fused into keep their registers for the streaming part)
     if (lane < 32 && c < n) {
-#pragma unroll 4
+#pragma unroll 1
         for (int r = 0; r < n; ++r) {
             double s = 0.0;
-            for (int k = r; k <= c; ++k) s = fmat(L.G[r][k], r1c[k], s);
+#pragma unroll 1
+            for (int k = r; k <= c; ++k) s = fmat(L.G[r][k], R1[k * 32 + c], s);
             R[r + (int64_t)c * ldr] = (r <= c) ? (float)s : 0.0f;
         }
     }
     double x[32];
     tri_inv_wave(L, lane, x);
     if (lane < 32) {
-        double* X = side + (int64_t)d.lv[0].tiles * 1024;
 #pragma unroll
         for (int r = 0; r < 32; ++r) X[r * 32 + c] = x[r];
         if (lane == 0) ctl[1] = 1;
     }
+}
+
+// Check kernel (one workgroup of 256 threads): G = sum of the nparts
+// partials (in order), then mixed_check_body with X = side block T0, R1 in
+// side block T0+1 and the residuals in side block T0+2.
+// (A template only so that every translation unit gets its own copy.)
+template <int UNUSED = 0>
+__global__ __launch_bounds__(256) void tsqr_mixed_check(const TreeDesc d, int nparts,
+                                                        const double* __restrict__ partial, float* __restrict__ R,
+                                                        int ldr, double* __restrict__ side, int* __restrict__ ctl,
+                                                        int pass, double tol) {
+    __shared__ GramLds L;
+    if (!ctl[0] || (pass == 2 && !ctl[1])) return;
+    const int t = threadIdx.x;
+    sum_partials(L, nparts, partial, t);
+    __syncthreads();
+    if (t >= 64) return;
+    const int64_t T0 = d.lv[0].tiles;
+    mixed_check_body(L, d.n, R, ldr, side + T0 * 1024, side + (T0 + 1) * 1024, side + (T0 + 2) * 1024, ctl, pass,
+                     tol, t);
 }
 
 }  // namespace tsqr

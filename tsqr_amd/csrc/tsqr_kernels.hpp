@@ -163,17 +163,14 @@ __device__ __forceinline__ int tile_rows_of(const LevelDesc& L, int tile) {
 // of a parent to finish (ticket) goes on to factor the parent, and so on up
 // to the root.  Nothing waits: a wave that is not last simply exits.  The
 // root writes R to the caller and its explicit Q (the S factors of its
-// children) over its block.
+// children) over its block.  tree_up_body does this for level-1 block
+// `tile`; the kernel below maps one block per workgroup.
 template <typename T, int N, int RPL>
-__global__ __launch_bounds__(64) void tsqr_tree_up(const TreeDesc d, T* __restrict__ R, int ldr,
-                                                    T* __restrict__ blocks, int* __restrict__ tickets,
-                                                    const int* __restrict__ skip) {
-    __shared__ TileLds<T, N> L;
-    if (skip[0]) return;
-    const int lane = threadIdx.x;
+__device__ __forceinline__ void tree_up_body(const TreeDesc& d, T* __restrict__ R, int ldr, T* __restrict__ blocks,
+                                             int* __restrict__ tickets, TileLds<T, N>& L, int tile) {
+    const int lane = threadIdx.x & 63;
     const int n = d.n;
     int level = 1;
-    int tile = blockIdx.x;
     int rows = tile_rows_of(d.lv[1], tile);
     T* blk = blocks + d.lv[1].blk_off + (int64_t)tile * d.H * n;
     T x[RPL][N];
@@ -212,6 +209,15 @@ __global__ __launch_bounds__(64) void tsqr_tree_up(const TreeDesc d, T* __restri
     store_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
 }
 
+template <typename T, int N, int RPL>
+__global__ __launch_bounds__(64) void tsqr_tree_up(const TreeDesc d, T* __restrict__ R, int ldr,
+                                                    T* __restrict__ blocks, int* __restrict__ tickets,
+                                                    const int* __restrict__ skip) {
+    __shared__ TileLds<T, N> L;
+    if (skip[0]) return;
+    tree_up_body<T, N, RPL>(d, R, ldr, blocks, tickets, L, blockIdx.x);
+}
+
 // A single tile (m <= H): factor, R to the caller, Q over A.
 template <typename T, int N, int RPL, bool VEC>
 __global__ __launch_bounds__(64) void tsqr_single(const TreeDesc d, T* __restrict__ A, int lda, T* __restrict__ R,
@@ -228,14 +234,14 @@ __global__ __launch_bounds__(64) void tsqr_single(const TreeDesc d, T* __restric
     store_rows<T, N, RPL, VEC>(x, A, lda, rows, n, lane);
 }
 
+// Down sweep of one tile of `level` (1 <= level <= root - 1): its explicit
+// Q times the S factor its parent holds for it, written over its block (the
+// S factors of its own children).
 template <typename T, int N, int RPL>
-__global__ __launch_bounds__(64) void tsqr_down(const TreeDesc d, int level, T* __restrict__ blocks,
-                                                 const int* __restrict__ skip) {
-    __shared__ TileLds<T, N> L;
-    if (skip[0]) return;
-    const int lane = threadIdx.x;
+__device__ __forceinline__ void down_body(const TreeDesc& d, int level, T* __restrict__ blocks, TileLds<T, N>& L,
+                                          int tile) {
+    const int lane = threadIdx.x & 63;
     const int n = d.n;
-    const int tile = blockIdx.x;
     const LevelDesc& Lv = d.lv[level];
     const LevelDesc& P = d.lv[level + 1];
     const int rows = tile_rows_of(Lv, tile);
@@ -247,6 +253,14 @@ __global__ __launch_bounds__(64) void tsqr_down(const TreeDesc d, int level, T* 
     const int slot = tile - parent * d.arity;
     const T* S = blocks + P.blk_off + (int64_t)parent * d.H * n + slot * n;
     apply_s_store<T, N, RPL, true>(x, S, d.H, rows < n ? rows : n, n, myblk, d.H, rows, L.R, lane);
+}
+
+template <typename T, int N, int RPL>
+__global__ __launch_bounds__(64) void tsqr_down(const TreeDesc d, int level, T* __restrict__ blocks,
+                                                 const int* __restrict__ skip) {
+    __shared__ TileLds<T, N> L;
+    if (skip[0]) return;
+    down_body<T, N, RPL>(d, level, blocks, L, blockIdx.x);
 }
 
 // One copy per translation unit (each dtype's TU launches its own).
