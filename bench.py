@@ -21,7 +21,12 @@ cpu_baseline  the oracle's C restatement of the reference algorithm
         (OpenMP over tiles, all host cores) on a bounded sample of the same
         workload (m/8 rows), plus OpenBLAS sgeqrf+sorgqr on the same sample.
 
-Other configurations (27648x32, 27648x128, FP64, batched, mixed) are
+batched  (same line, key "batched") the headline's second half: QRs/s of the
+        512-matrix uneven batch, mixed method vs plain FP32 Householder, each
+        as one hipGraph replay per batch, with the max backward and
+        orthogonality errors of each method over the batch.
+
+Other configurations (27648x32, 27648x128, FP64, mixed single-matrix) are
 exercised by the parity tests; --workload selects them for ad-hoc runs.
 """
 from __future__ import annotations
@@ -73,7 +78,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="tsqr_max_fp32", choices=sorted(WORKLOADS))
+    p.add_argument("--workload", default="tsqr_max_fp32", choices=sorted(WORKLOADS) + ["batch_512_fp32"])
     p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     p.add_argument("--cpu-fraction", type=int, default=8, help="cpu sample = m / this")
     p.add_argument("--copy-bytes", type=int, default=0,
@@ -81,7 +86,68 @@ def parse():
                         "known size for calibrating memory counters) and report its bandwidth")
     p.add_argument("--leaf-path", choices=("default", "auto", "householder", "gram_tree"), default="default",
                    help="level-0 superleaf path (default: the library's default for the dtype)")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched (mixed vs plain FP32) leg")
+    p.add_argument("--batch-steps", type=int, default=5)
+    p.add_argument("--batch-count", type=int, default=512)
     return p.parse_args()
+
+
+def batched_leg(steps: int, count: int) -> dict:
+    """512 uneven FP32 matrices (m log-uniform in [4096, 262144], a multiple
+    of 64; n in {8, 16, 32, 64}; iid N(0,1) from a seeded torch generator),
+    factored by one replay of the captured hipGraph per step, mixed method
+    and plain (Householder) method.  Each step starts from the same input
+    (an untimed device copy restores it); the step is timed with HIP events
+    on the launch stream.  Errors of the last step of each method are measured
+    in FP64 on the GPU against the input: max over the batch of
+    ||A - QR||_F / ||A||_F and ||I - Q^T Q||_F."""
+    import torch
+
+    import tsqr_amd
+    from tsqr_amd.workloads import batch_shapes
+
+    dev = torch.device("cuda", 0)
+    shapes = batch_shapes(count=count)
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    src = [tsqr_amd.empty_colmajor(m, n, torch.float32, dev) for m, n in shapes]
+    for a in src:
+        a.normal_(generator=g)
+    mats = [a.clone(memory_format=torch.preserve_format) for a in src]
+    mats = [tsqr_amd.colmajor(a) if a.stride(0) != 1 else a for a in mats]
+    balg = sum((3 * m * n + n * n) * 4 for m, n in shapes)
+    stream = torch.cuda.current_stream(dev)
+    out = {"workload": f"{count} matrices, m log-uniform in [4096, 262144] (multiple of 64), n in {{8,16,32,64}}, "
+                       f"float32 iid N(0,1); one hipGraph replay per batch",
+           "matrices": count, "algorithmic_bytes_per_batch": balg}
+    for method in ("mixed", "plain"):
+        b = tsqr_amd.Batch(mats, method=method)
+        times = []
+        for k in range(steps + 1):  # the first replay (graph capture) is untimed
+            for a, s_ in zip(mats, src):
+                a.copy_(s_)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            b.graph_launch(stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if k:
+                times.append(e0.elapsed_time(e1))
+        st = b.stats()
+        back = orth = 0.0
+        for a, q, r in zip(src, mats, b.Rs):
+            a64, q64, r64 = a.double(), q.double(), r.double()
+            back = max(back, float(torch.linalg.norm(a64 - q64 @ r64) / torch.linalg.norm(a64)))
+            eye = torch.eye(q.shape[1], dtype=torch.float64, device=dev)
+            orth = max(orth, float(torch.linalg.norm(eye - q64.T @ q64)))
+        ms = sum(times) / len(times)
+        out[method] = {"ms_per_batch": round(ms, 4), "qrs_per_s": round(count / (ms / 1e3), 1),
+                       "GBs": round(balg / (ms / 1e3) / 1e9, 1), "max_backward": float(f"{back:.3e}"),
+                       "max_orth": float(f"{orth:.3e}"), "panels": st}
+        b.close()
+        print(f"# batched {method}: {out[method]}", file=sys.stderr, flush=True)
+    out["mixed_over_plain_qrs"] = round(out["mixed"]["qrs_per_s"] / out["plain"]["qrs_per_s"], 3)
+    return out
 
 
 def cpu_baseline(m: int, n: int, dtype: str, frac: int) -> dict:
@@ -141,6 +207,17 @@ def main():
     import torch
 
     import tsqr_amd
+
+    if args.workload == "batch_512_fp32":
+        # the batched leg alone: value = mixed-method QRs/s
+        torch.cuda.set_device(0)
+        b = batched_leg(args.steps, args.batch_count)
+        print(json.dumps({"metric": "batched QR/s mixed vs FP32 at equal error", "value": b["mixed"]["qrs_per_s"],
+                          "unit": "QR/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+                          "ms_per_step": b["mixed"]["ms_per_batch"], "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f32", "data": "synthetic: iid N(0,1), seeded",
+                          "config": {"workload": b["workload"]}, "batched": b}), flush=True)
+        return
     from tsqr_amd._lib import (TSQR_DOUBLE, TSQR_FLOAT, TSQR_LEAF_AUTO, TSQR_LEAF_GRAM_TREE,
                                TSQR_LEAF_HOUSEHOLDER, leaf_stats, mixed_stats, set_leaf_path, timing_read)
 
@@ -301,6 +378,8 @@ def main():
         },
         "roofline": roofline,
     }
+    if not args.no_batch and args.workload == "tsqr_max_fp32":
+        out["batched"] = batched_leg(args.batch_steps, args.batch_count)
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(m, n, dtype, args.cpu_fraction)

@@ -53,6 +53,8 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
          const int* ldr) {
     std::vector<BatchProb> first, second;
     std::vector<BatchCouple> couples;
+    // the workspace starts with the two stages' Householder fallback lists
+    // (count, pad, then one int2 per superleaf); sized after planning
     size_t off = 0;
     std::vector<int> owner_second;  // matrix index of each second panel
     for (int i = 0; i < b->count; ++i) {
@@ -87,6 +89,13 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
     }
     std::vector<BatchProb> all(first);
     all.insert(all.end(), second.begin(), second.end());
+    size_t leaves_in[2] = {0, 0};
+    for (size_t k = 0; k < all.size(); ++k) leaves_in[k < first.size() ? 0 : 1] += all[k].d.lv[0].tiles;
+    const size_t list_bytes[2] = {align256(8 + leaves_in[0] * sizeof(int2)), align256(8 + leaves_in[1] * sizeof(int2))};
+    const size_t head = list_bytes[0] + list_bytes[1];
+    for (auto& p : all) p.ws += (int64_t)head;
+    for (auto& c : couples) c.ws += (int64_t)head;
+    off += head;
 
     // item lists (problem indices into `all`)
     struct Stage {
@@ -159,6 +168,7 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
     dev.ncouples = (int)couples.size();
     dev.couple_rows = list(a_crow);
     for (int s = 0; s < 2; ++s) {
+        dev.stage[s].fallback = s == 0 ? 0 : (int64_t)list_bytes[0];
         dev.stage[s].probs = list(a_probs[s]);
         dev.stage[s].leaves = list(a_leaves[s]);
         dev.stage[s].level1 = list(a_l1[s]);

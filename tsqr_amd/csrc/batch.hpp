@@ -8,8 +8,10 @@
 //                   32 < n <= 64 is two: columns [0, 32) and [32, n);
 //   couples         for the latter, the coupling of the two panels:
 //                   R12 = Q1^T A2 and A2 -= Q1 R12 (block Gram-Schmidt,
-//                   block_QR.h:88-93), done once for the plain method and
-//                   twice (BCGS2) for the mixed one;
+//                   block_QR.h:88-93), done once for the plain method; the
+//                   mixed method repeats it (BCGS2) for the matrices where
+//                   the first projection removed more than half of some
+//                   column's squared norm;
 //   item lists      (problem, index) pairs: every superleaf of every panel,
 //                   every level-1 tree block, every tile of each tree level,
 //                   every row block of every couple.
@@ -37,6 +39,7 @@
 #pragma once
 #include <type_traits>
 
+#include "gemm_mfma.hpp"
 #include "mixed.hpp"
 #include "super.hpp"
 
@@ -65,10 +68,17 @@ struct BatchCouple {
     int vec, pad;
 };
 
-// Per-couple workspace: block b < leaves is superleaf b's product, block
+// Per-couple workspace: block b < leaves is row block b's product, block
 // `leaves` the summed product P (the projection's right factor), block
-// `leaves + 1` the accumulated R12 in FP64; then the tickets.
-inline size_t couple_ws_bytes(int leaves) { return align256(((size_t)leaves + 2) * 1024 * sizeof(double) + 64); }
+// `leaves + 1` the accumulated R12 in FP64; then the squared column norms of
+// A2 per row block (32 doubles each); then 16 ints: tickets of the two
+// passes and the reorthogonalisation flag.
+__host__ __device__ inline size_t couple_norms_off(int leaves) { return ((size_t)leaves + 2) * 1024 * sizeof(double); }
+__host__ __device__ inline size_t couple_ints_off(int leaves) {
+    return couple_norms_off(leaves) + (size_t)leaves * 32 * sizeof(double);
+}
+inline size_t couple_ws_bytes(int leaves) { return align256(couple_ints_off(leaves) + 64); }
+constexpr int kCoupleReorth = 4;  // int index of the flag
 
 template <int V>
 using IntC = std::integral_constant<int, V>;
@@ -145,13 +155,19 @@ constexpr int kCtlGramTicket = 8;
 
 // ---------------------------------------------------------------------------
 // Problem set-up: tickets and control words = 0; couples' tickets = 0;
-// R21 = 0 for the two-panel matrices.  One workgroup per problem / couple.
+// R21 = 0 for the two-panel matrices; the two Householder fallback lists
+// (count, pad, then int2 items, at byte offsets list0 / list1) empty.  One
+// workgroup per problem / couple.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void batch_init(const BatchProb* __restrict__ probs, int nprobs,
                                                    const BatchCouple* __restrict__ couples, int ncouples,
-                                                   char* __restrict__ work) {
+                                                   char* __restrict__ work, int64_t list0, int64_t list1) {
     const int b = blockIdx.x;
+    if (b == 0 && threadIdx.x == 0) {
+        *reinterpret_cast<int*>(work + list0) = 0;
+        *reinterpret_cast<int*>(work + list1) = 0;
+    }
     if (b < nprobs) {
         const BatchProb& p = probs[b];
         char* ws = prob_ws(p, work);
@@ -164,7 +180,7 @@ __global__ __launch_bounds__(256) void batch_init(const BatchProb* __restrict__ 
     const int c = b - nprobs;
     if (c >= ncouples) return;
     const BatchCouple& q = couples[c];
-    int* tk = reinterpret_cast<int*>(work + q.ws + ((size_t)q.leaves + 2) * 1024 * sizeof(double));
+    int* tk = reinterpret_cast<int*>(work + q.ws + couple_ints_off(q.leaves));
     if (threadIdx.x < 16) tk[threadIdx.x] = 0;
     // R21: rows [32, 32 + n2) of columns [0, 32), i.e. relative to R12 at
     // (0, 32): rows 32.., columns -32..-1
@@ -176,11 +192,8 @@ __global__ __launch_bounds__(256) void batch_init(const BatchProb* __restrict__ 
 // Householder superleaves (plain problems, and mixed ones whose gate failed).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(batch_super_threads<T>()) void batch_super_r(const BatchProb* __restrict__ probs,
-                                                                         const int2* __restrict__ items,
-                                                                         char* __restrict__ work) {
-    __shared__ __align__(16) char smem[super_lds_bytes<T>()];
-    const int2 it = items[blockIdx.x];
+__device__ __forceinline__ void super_r_item(const BatchProb* __restrict__ probs, int2 it, char* __restrict__ work,
+                                             char* smem) {
     const BatchProb& p = probs[it.x];
     char* ws = prob_ws(p, work);
     if (reinterpret_cast<const int*>(ws + p.w.ctl)[0]) return;
@@ -196,11 +209,8 @@ __global__ __launch_bounds__(batch_super_threads<T>()) void batch_super_r(const 
 }
 
 template <typename T>
-__global__ __launch_bounds__(batch_super_threads<T>()) void batch_super_q(const BatchProb* __restrict__ probs,
-                                                                         const int2* __restrict__ items,
-                                                                         char* __restrict__ work) {
-    __shared__ __align__(16) char smem[super_lds_bytes<T>()];
-    const int2 it = items[blockIdx.x];
+__device__ __forceinline__ void super_q_item(const BatchProb* __restrict__ probs, int2 it, char* __restrict__ work,
+                                             char* smem) {
     const BatchProb& p = probs[it.x];
     char* ws = prob_ws(p, work);
     if (reinterpret_cast<const int*>(ws + p.w.ctl)[0]) return;
@@ -213,6 +223,43 @@ __global__ __launch_bounds__(batch_super_threads<T>()) void batch_super_q(const 
         T* stash = reinterpret_cast<T*>(smem + sizeof(SuperLds<T, N, RPL>));
         super_q_leaf<T, N, RPL, VEC>(p.d, reinterpret_cast<T*>(p.A), p.lda, blocks, S, stash, it.y);
     });
+}
+
+// Both kernels take either a fixed item list (LIST = false: one item per
+// workgroup) or a list built on the device (LIST = true: *count items,
+// grid-stride), the superleaves of the mixed problems whose gate failed.
+template <typename T, bool LIST>
+__global__ __launch_bounds__(batch_super_threads<T>()) void batch_super_r(const BatchProb* __restrict__ probs,
+                                                                         const int2* __restrict__ items,
+                                                                         const int* __restrict__ count,
+                                                                         char* __restrict__ work) {
+    __shared__ __align__(16) char smem[super_lds_bytes<T>()];
+    if constexpr (LIST) {
+#pragma unroll 1
+        for (int i = blockIdx.x; i < count[0]; i += gridDim.x) {
+            super_r_item<T>(probs, items[i], work, smem);
+            __syncthreads();
+        }
+    } else {
+        super_r_item<T>(probs, items[blockIdx.x], work, smem);
+    }
+}
+
+template <typename T, bool LIST>
+__global__ __launch_bounds__(batch_super_threads<T>()) void batch_super_q(const BatchProb* __restrict__ probs,
+                                                                         const int2* __restrict__ items,
+                                                                         const int* __restrict__ count,
+                                                                         char* __restrict__ work) {
+    __shared__ __align__(16) char smem[super_lds_bytes<T>()];
+    if constexpr (LIST) {
+#pragma unroll 1
+        for (int i = blockIdx.x; i < count[0]; i += gridDim.x) {
+            super_q_item<T>(probs, items[i], work, smem);
+            __syncthreads();
+        }
+    } else {
+        super_q_item<T>(probs, items[blockIdx.x], work, smem);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -261,7 +308,7 @@ __device__ __forceinline__ double* prob_side(const BatchProb& p, char* work) {
 template <int UNUSED = 0>
 __global__ __launch_bounds__(64) void batch_mixed_gram(const BatchProb* __restrict__ probs,
                                                         const int2* __restrict__ items, char* __restrict__ work,
-                                                        double gate) {
+                                                        double gate, int* __restrict__ fallback) {
     __shared__ double sums[12 * 64];
     __shared__ GramLds L;
     const int lane = threadIdx.x;
@@ -280,8 +327,16 @@ __global__ __launch_bounds__(64) void batch_mixed_gram(const BatchProb* __restri
     if (!last_arriver(ctl + kCtlGramTicket, L0.tiles, lane)) return;
     sum_blocks_wave(L, side, L0.tiles, lane);
     const int64_t T0 = L0.tiles;
-    flat_factor<float>(L, n, reinterpret_cast<float*>(p.R), p.ldr, side + T0 * 1024, ctl, gate,
-                       side + (T0 + 1) * 1024, lane);
+    const bool ok = flat_factor<float>(L, n, reinterpret_cast<float*>(p.R), p.ldr, side + T0 * 1024, ctl, gate,
+                                       side + (T0 + 1) * 1024, lane);
+    // gate failed: this matrix's superleaves go to the Householder list
+    if (!ok) {
+        int at = 0;
+        if (lane == 0) at = __hip_atomic_fetch_add(fallback, L0.tiles, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        at = __builtin_amdgcn_readfirstlane(at);
+        int2* dst = reinterpret_cast<int2*>(fallback + 2) + at;
+        for (int t = lane; t < L0.tiles; t += 64) dst[t] = make_int2(it.x, t);
+    }
 }
 
 // (at least two waves per SIMD: the register budget of the single-matrix
@@ -330,8 +385,14 @@ __global__ __launch_bounds__(64) void batch_mixed_check(const BatchProb* __restr
 // or FP64 inputs exact in FP64), summed by the last row block into R12;
 // then A2 -= Q1 P.  `pass` 2 (mixed method) adds the second product to R12.
 // ---------------------------------------------------------------------------
+
+// P_b = Q1_b^T A2_b for row block `blk`, written to out (FP64 32 x 32,
+// row-major).  Lane l loads rows 4(l/16)..+3 of columns l%16 and 16 + l%16
+// of both matrices; batches of D 16-row slabs are loaded together and the
+// next batch is in flight while the current one is multiplied.
 template <typename T, bool VEC>
-__device__ __forceinline__ void cross_block(const BatchCouple& q, int blk, double* __restrict__ out, int lane) {
+__device__ __forceinline__ void cross_block(const BatchCouple& q, int blk, double* __restrict__ out,
+                                            double* __restrict__ norms, int lane) {
     const int col = lane & 15;
     const int rq = 4 * (lane >> 4);
     const int64_t r0 = (int64_t)blk * q.leaf_rows;
@@ -340,21 +401,63 @@ __device__ __forceinline__ void cross_block(const BatchCouple& q, int blk, doubl
     const T* Qb = reinterpret_cast<const T*>(q.Q1) + r0;
     const T* Ab = reinterpret_cast<const T*>(q.A2) + r0;
     v4d c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
-#pragma unroll 1
-    for (int s0 = 0; s0 < rows; s0 += 16) {
-        double q0[4], q1[4], a0[4], a1[4];
-        load4<T, VEC>(q0, Qb, q.lda, s0 + rq, col, rows, 32);
-        load4<T, VEC>(q1, Qb, q.lda, s0 + rq, col + 16, rows, 32);
-        load4<T, VEC>(a0, Ab, q.lda, s0 + rq, col, rows, q.n2);
-        load4<T, VEC>(a1, Ab, q.lda, s0 + rq, col + 16, rows, q.n2);
+    double s0sq = 0.0, s1sq = 0.0;  // this lane's part of the squared column norms of A2
+    auto mma4 = [&](const Raw4<T>& q0, const Raw4<T>& q1, const Raw4<T>& a0, const Raw4<T>& a1) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(q0[t], a0[t], c00, 0, 0, 0);
-            c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(q0[t], a1[t], c01, 0, 0, 0);
-            c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(q1[t], a0[t], c10, 0, 0, 0);
-            c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(q1[t], a1[t], c11, 0, 0, 0);
+            const double x0 = (double)q0.v[t], x1 = (double)q1.v[t];
+            const double y0 = (double)a0.v[t], y1 = (double)a1.v[t];
+            s0sq = fmat(y0, y0, s0sq);
+            s1sq = fmat(y1, y1, s1sq);
+            c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y0, c00, 0, 0, 0);
+            c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y1, c01, 0, 0, 0);
+            c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y0, c10, 0, 0, 0);
+            c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y1, c11, 0, 0, 0);
+        }
+    };
+    constexpr int D = 4;
+    int s0 = 0;
+    if (VEC && rows >= D * 16) {
+        Raw4<T> cq0[D], cq1[D], ca0[D], ca1[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            cq0[k] = load_raw4<T, true>(Qb, q.lda, 16 * k + rq, col, rows, 32);
+            cq1[k] = load_raw4<T, true>(Qb, q.lda, 16 * k + rq, col + 16, rows, 32);
+            ca0[k] = load_raw4<T, true>(Ab, q.lda, 16 * k + rq, col, rows, q.n2);
+            ca1[k] = load_raw4<T, true>(Ab, q.lda, 16 * k + rq, col + 16, rows, q.n2);
+        }
+#pragma unroll 1
+        for (;;) {
+            const int next = s0 + D * 16;
+            const bool more = next + D * 16 <= rows;
+            Raw4<T> pq0[D], pq1[D], pa0[D], pa1[D];
+            if (more) {
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    pq0[k] = load_raw4<T, true>(Qb, q.lda, next + 16 * k + rq, col, rows, 32);
+                    pq1[k] = load_raw4<T, true>(Qb, q.lda, next + 16 * k + rq, col + 16, rows, 32);
+                    pa0[k] = load_raw4<T, true>(Ab, q.lda, next + 16 * k + rq, col, rows, q.n2);
+                    pa1[k] = load_raw4<T, true>(Ab, q.lda, next + 16 * k + rq, col + 16, rows, q.n2);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < D; ++k) mma4(cq0[k], cq1[k], ca0[k], ca1[k]);
+            s0 = next;
+            if (!more) break;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                cq0[k] = pq0[k];
+                cq1[k] = pq1[k];
+                ca0[k] = pa0[k];
+                ca1[k] = pa1[k];
+            }
         }
     }
+#pragma unroll 1
+    for (; s0 < rows; s0 += 16)
+        mma4(load_raw4<T, VEC>(Qb, q.lda, s0 + rq, col, rows, 32), load_raw4<T, VEC>(Qb, q.lda, s0 + rq, col + 16, rows, 32),
+             load_raw4<T, VEC>(Ab, q.lda, s0 + rq, col, rows, q.n2),
+             load_raw4<T, VEC>(Ab, q.lda, s0 + rq, col + 16, rows, q.n2));
     // f64 accumulator: lane l, register v holds C[4v + l/16][l%16]
     const int i0 = lane >> 4;
 #pragma unroll
@@ -365,8 +468,21 @@ __device__ __forceinline__ void cross_block(const BatchCouple& q, int blk, doubl
         out[(16 + i) * 32 + col] = c10[v];
         out[(16 + i) * 32 + 16 + col] = c11[v];
     }
+    // column sums over the four row groups (lane bits 4 and 5)
+    s0sq = fold_all<0>(fold_all<1>(s0sq));
+    s1sq = fold_all<0>(fold_all<1>(s1sq));
+    if (lane < 16) {
+        norms[col] = s0sq;
+        norms[16 + col] = s1sq;
+    }
 }
 
+// Pass 1 also decides whether the second pass is needed ("twice is enough",
+// Kahan-Parlett): with ||a_j||^2 the squared norm of column j of A2 and
+// P_j the product's column j, ||a_j - Q1 P_j||^2 = ||a_j||^2 - ||P_j||^2
+// (Q1 orthonormal); if every column keeps at least half its squared norm,
+// one projection leaves A2 orthogonal to Q1 to the working precision and the
+// second pass is skipped.  (The plain method never runs a second pass.)
 template <typename T>
 __global__ __launch_bounds__(64) void batch_cross(const BatchCouple* __restrict__ couples,
                                                    const int2* __restrict__ items, char* __restrict__ work,
@@ -376,11 +492,13 @@ __global__ __launch_bounds__(64) void batch_cross(const BatchCouple* __restrict_
     const int2 it = items[blockIdx.x];
     const BatchCouple& q = couples[it.x];
     double* blocks = reinterpret_cast<double*>(work + q.ws);
-    int* tk = reinterpret_cast<int*>(work + q.ws + ((size_t)q.leaves + 2) * 1024 * sizeof(double));
+    double* norms = reinterpret_cast<double*>(work + q.ws + couple_norms_off(q.leaves));
+    int* tk = reinterpret_cast<int*>(work + q.ws + couple_ints_off(q.leaves));
+    if (pass == 2 && !tk[kCoupleReorth]) return;
     if (q.vec)
-        cross_block<T, true>(q, it.y, blocks + (int64_t)it.y * 1024, lane);
+        cross_block<T, true>(q, it.y, blocks + (int64_t)it.y * 1024, norms + (int64_t)it.y * 32, lane);
     else
-        cross_block<T, false>(q, it.y, blocks + (int64_t)it.y * 1024, lane);
+        cross_block<T, false>(q, it.y, blocks + (int64_t)it.y * 1024, norms + (int64_t)it.y * 32, lane);
     if (!last_arriver(tk + pass - 1, q.leaves, lane)) return;
     sum_blocks_wave(L, blocks, q.leaves, lane);
     double* P = blocks + (int64_t)q.leaves * 1024;
@@ -396,68 +514,93 @@ __global__ __launch_bounds__(64) void batch_cross(const BatchCouple* __restrict_
         Racc[idx] = r;
         if (j < q.n2) R12[i + (int64_t)j * q.ldr] = (T)r;
     }
+    if (pass == 1) {
+        // lane j < 32: ||a_j||^2 (row blocks in order) against ||P_j||^2
+        const int j = lane & 31;
+        double an = 0.0, pn = 0.0;
+#pragma unroll 1
+        for (int b = 0; b < q.leaves; ++b) an += norms[(int64_t)b * 32 + j];
+#pragma unroll 4
+        for (int i = 0; i < 32; ++i) pn = fmat(L.G[i][j], L.G[i][j], pn);
+        const bool lost = j < q.n2 && !(2.0 * pn <= an);  // NaN counts as lost
+        const uint64_t any = __ballot(lost);
+        if (lane == 0) tk[kCoupleReorth] = any ? 1 : 0;
+    }
 }
 
-// A2 rows of one block -= Q1 rows * P (32 x n2, FP64): FP64 MFMA with the
-// accumulators initialised from A2 and B = -P, one rounding.
+// A2 rows of one block -= Q1 rows * P on the matrix cores in the working
+// precision (v_mfma_f32_16x16x4f32 for float, v_mfma_f64_16x16x4f64 for
+// double), accumulators initialised from A2 and B = -P, one rounding.
+// Rows in blocks of 64; lane l feeds row 4(l%16) + t of column 4 kg + l/16
+// to MFMA t, so output register v of (t, jt) is row Mfma16<T>::row(v, l/16)
+// * 4 + t (16(l/16) + 4v + t for float, 16v + 4(l/16) + t for double),
+// column 16 jt + l%16, and each lane stores 4 consecutive rows per (v, jt).
+// The next block's loads are issued before the current block's MFMAs.
 template <typename T, bool VEC>
 __device__ __forceinline__ void project_block(const BatchCouple& q, int blk, const double* __restrict__ P, int lane) {
+    using M = Mfma16<T>;
     const int li = lane & 15, lq = lane >> 4;
     const int64_t r0 = (int64_t)blk * q.leaf_rows;
     const int64_t left = q.m - r0;
     const int rows = (int)(left < q.leaf_rows ? left : q.leaf_rows);
     const T* Qb = reinterpret_cast<const T*>(q.Q1) + r0;
     T* Ab = reinterpret_cast<T*>(q.A2) + r0;
-    double mb[8][2];  // -P in the B-operand layout of k-group kg (k = 4 kg + l/16)
+    T mb[8][2];  // -P in the B-operand layout of k-group kg (k = 4 kg + l/16)
 #pragma unroll
     for (int kg = 0; kg < 8; ++kg)
 #pragma unroll
-        for (int jt = 0; jt < 2; ++jt) mb[kg][jt] = -P[(4 * kg + lq) * 32 + 16 * jt + li];
-#pragma unroll 1
-    for (int s0 = 0; s0 < rows; s0 += 64) {
-        // acc[t][jt] register v: row s0 + 16v + 4 lq + t, column 16 jt + li
-        v4d acc[4][2];
+        for (int jt = 0; jt < 2; ++jt) mb[kg][jt] = (T)(-P[(4 * kg + lq) * 32 + 16 * jt + li]);
+    auto row_of = [&](int v) { return 4 * M::row(v, lq); };  // first of the 4 rows of register v
+    Four<T> qa[8], cv[2][4];
+    auto load_block = [&](int s0) {
+#pragma unroll
+        for (int kg = 0; kg < 8; ++kg) qa[kg] = load_four<T, VEC>(Qb, q.lda, s0 + 4 * li, 4 * kg + lq, rows, 32);
 #pragma unroll
         for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                double a[4];
-                load4<T, VEC>(a, Ab, q.lda, s0 + 16 * v + 4 * lq, 16 * jt + li, rows, q.n2);
+            for (int v = 0; v < 4; ++v)
+                cv[jt][v] = load_four<T, VEC>(Ab, q.lda, s0 + row_of(v), 16 * jt + li, rows, q.n2);
+    };
+    if (rows > 0) load_block(0);
+#pragma unroll 1
+    for (int s0 = 0; s0 < rows; s0 += 64) {
+        typename M::v4 acc[4][2];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) acc[t][jt][v] = a[t];
-            }
+        for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
-        for (int kg = 0; kg < 8; ++kg) {
-            double x[4];
-            load4<T, VEC>(x, Qb, q.lda, s0 + 4 * li, 4 * kg + lq, rows, 32);
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[t][jt][v] = cv[jt][v].v[t];
+        Four<T> x[8];
+#pragma unroll
+        for (int kg = 0; kg < 8; ++kg) x[kg] = qa[kg];
+        if (s0 + 64 < rows) load_block(s0 + 64);
+#pragma unroll
+        for (int kg = 0; kg < 8; ++kg)
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int jt = 0; jt < 2; ++jt)
-                    acc[t][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[t], mb[kg][jt], acc[t][jt], 0, 0, 0);
-        }
+                for (int jt = 0; jt < 2; ++jt) acc[t][jt] = M::mma(x[kg].v[t], mb[kg][jt], acc[t][jt]);
 #pragma unroll
         for (int jt = 0; jt < 2; ++jt) {
-            const int cc = 16 * jt + li;
-            if (cc < q.n2) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    T o[4];
+            for (int v = 0; v < 4; ++v) {
+                Four<T> o;
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) o[t] = (T)acc[t][jt][v];
-                    store_column<T, 4, VEC>(o, Ab + (int64_t)cc * q.lda + s0 + 16 * v + 4 * lq, rows,
-                                            (int)(s0 + 16 * v + 4 * lq));
-                }
+                for (int t = 0; t < 4; ++t) o.v[t] = acc[t][jt][v];
+                store_four<T, VEC>(o, Ab, q.lda, s0 + row_of(v), 16 * jt + li, rows, q.n2);
             }
         }
     }
 }
 
 template <typename T>
-__global__ __launch_bounds__(64) void batch_project(const BatchCouple* __restrict__ couples,
-                                                     const int2* __restrict__ items, char* __restrict__ work) {
+__global__ __launch_bounds__(64, 2) void batch_project(const BatchCouple* __restrict__ couples,
+                                                        const int2* __restrict__ items, char* __restrict__ work,
+                                                        int pass) {
     const int2 it = items[blockIdx.x];
     const BatchCouple& q = couples[it.x];
+    if (pass == 2 && !reinterpret_cast<const int*>(work + q.ws + couple_ints_off(q.leaves))[kCoupleReorth]) return;
     const double* P = reinterpret_cast<const double*>(work + q.ws) + (int64_t)q.leaves * 1024;
     if (q.vec)
         project_block<T, true>(q, it.y, P, threadIdx.x);

@@ -11,13 +11,23 @@ template <typename T>
 void launch_panel_stage(const BatchDev& b, const BatchStage& s, char* work, hipStream_t st) {
     const bool mixed = sizeof(T) == 4 && b.method == 1;
     if (s.leaves.count == 0) return;
+    int* fallback = reinterpret_cast<int*>(work + s.fallback);
+    // Householder superleaves: every one (plain), or the device-built list of
+    // the mixed problems whose gate failed (grid-stride, usually empty)
+    const int2* hh_items = mixed ? reinterpret_cast<const int2*>(fallback + 2) : s.leaves.items;
+    const int* hh_count = mixed ? fallback : nullptr;
+    const int hh_grid = mixed ? (s.leaves.count < 1024 ? s.leaves.count : 1024) : s.leaves.count;
     if constexpr (sizeof(T) == 4) {
         if (mixed)
             hipLaunchKernelGGL(batch_mixed_gram<>, dim3(s.leaves.count), dim3(64), 0, st, b.probs, s.leaves.items,
-                               work, b.gate);
+                               work, b.gate, fallback);
     }
-    hipLaunchKernelGGL(batch_super_r<T>, dim3(s.leaves.count), dim3(batch_super_threads<T>()), 0, st, b.probs,
-                       s.leaves.items, work);
+    if (mixed)
+        hipLaunchKernelGGL((batch_super_r<T, true>), dim3(hh_grid), dim3(batch_super_threads<T>()), 0, st, b.probs,
+                           hh_items, hh_count, work);
+    else
+        hipLaunchKernelGGL((batch_super_r<T, false>), dim3(hh_grid), dim3(batch_super_threads<T>()), 0, st, b.probs,
+                           hh_items, hh_count, work);
     if (s.level1.count)
         hipLaunchKernelGGL(batch_tree_up<T>, dim3(s.level1.count), dim3(64), 0, st, b.probs, s.level1.items, work);
     for (int l = (int)s.down.size() - 1; l >= 1; --l)
@@ -33,8 +43,12 @@ void launch_panel_stage(const BatchDev& b, const BatchStage& s, char* work, hipS
                                    work, pass, b.tol);
             }
     }
-    hipLaunchKernelGGL(batch_super_q<T>, dim3(s.leaves.count), dim3(batch_super_threads<T>()), 0, st, b.probs,
-                       s.leaves.items, work);
+    if (mixed)
+        hipLaunchKernelGGL((batch_super_q<T, true>), dim3(hh_grid), dim3(batch_super_threads<T>()), 0, st, b.probs,
+                           hh_items, hh_count, work);
+    else
+        hipLaunchKernelGGL((batch_super_q<T, false>), dim3(hh_grid), dim3(batch_super_threads<T>()), 0, st, b.probs,
+                           hh_items, hh_count, work);
 }
 }  // namespace
 
@@ -45,7 +59,7 @@ int batch_launch(const BatchDev& b, void* work_v, hipStream_t st) {
     const int ninit = b.nprobs + b.ncouples;
     if (ninit)
         hipLaunchKernelGGL(batch_init<T>, dim3(ninit), dim3(256), 0, st, b.probs, b.nprobs, b.couples, b.ncouples,
-                           work);
+                           work, b.stage[0].fallback, b.stage[1].fallback);
     launch_panel_stage<T>(b, b.stage[0], work, st);
     if (b.ncouples) {
         const int passes = b.method == 1 ? 2 : 1;
@@ -53,7 +67,7 @@ int batch_launch(const BatchDev& b, void* work_v, hipStream_t st) {
             hipLaunchKernelGGL(batch_cross<T>, dim3(b.couple_rows.count), dim3(64), 0, st, b.couples,
                                b.couple_rows.items, work, pass);
             hipLaunchKernelGGL(batch_project<T>, dim3(b.couple_rows.count), dim3(64), 0, st, b.couples,
-                               b.couple_rows.items, work);
+                               b.couple_rows.items, work, pass);
         }
     }
     launch_panel_stage<T>(b, b.stage[1], work, st);

@@ -519,9 +519,10 @@ __device__ __forceinline__ void sum_partials(GramLds& L, int nparts, const doubl
 // 0..63; both half-waves factor the same matrix).  If it succeeds and
 // ||R||_F ||R^{-1}||_F <= gate: R (rounded to T) goes to the caller, R^{-1}
 // (FP64, row-major) to X, R itself in FP64 to rkeep (if not null) and
-// ctl[0] = 1.  Otherwise nothing is written.
+// ctl[0] = 1.  Otherwise nothing is written.  Returns whether the gate
+// passed (the same in every lane).
 template <typename T>
-__device__ __forceinline__ void flat_factor(GramLds& L, int n, T* __restrict__ R, int ldr, double* __restrict__ X,
+__device__ __forceinline__ bool flat_factor(GramLds& L, int n, T* __restrict__ R, int ldr, double* __restrict__ X,
                                             int* __restrict__ ctl, double gate, double* __restrict__ rkeep,
                                             int lane) {
     const int c = lane & 31;
@@ -546,7 +547,7 @@ __device__ __forceinline__ void flat_factor(GramLds& L, int n, T* __restrict__ R
     }
     const double kest = __builtin_sqrt(sum_half(nr) * sum_half(nx));
     ok = ok && (kest <= gate);
-    if (!ok || lane >= 32) return;
+    if (!ok || lane >= 32) return ok;
     if (c < n) {
 #pragma unroll 4
         for (int r = 0; r < n; ++r) R[r + (int64_t)c * ldr] = (r <= c) ? (T)L.G[r][c] : T(0);
@@ -558,6 +559,7 @@ __device__ __forceinline__ void flat_factor(GramLds& L, int n, T* __restrict__ R
 #pragma unroll
     for (int r = 0; r < 32; ++r) X[r * 32 + c] = x[r];
     if (c == 0) ctl[0] = 1;
+    return ok;
 }
 
 // Flat Gram path, stage 2 (one workgroup of 256 threads): G = sum of the
