@@ -84,7 +84,7 @@ def parse():
     p.add_argument("--copy-bytes", type=int, default=0,
                    help="before timing, copy a buffer of this many bytes device-to-device (a plain copy of "
                         "known size for calibrating memory counters) and report its bandwidth")
-    p.add_argument("--leaf-path", choices=("default", "auto", "householder", "gram_tree"), default="default",
+    p.add_argument("--leaf-path", choices=("default", "auto", "householder", "gram_tree", "cholqr2"), default="default",
                    help="level-0 superleaf path (default: the library's default for the dtype)")
     p.add_argument("--no-batch", action="store_true", help="skip the batched (mixed vs plain FP32) leg")
     p.add_argument("--batch-steps", type=int, default=5)
@@ -218,7 +218,7 @@ def main():
                           "vs_baseline": None, "dtype": "f32", "data": "synthetic: iid N(0,1), seeded",
                           "config": {"workload": b["workload"]}, "batched": b}), flush=True)
         return
-    from tsqr_amd._lib import (TSQR_DOUBLE, TSQR_FLOAT, TSQR_LEAF_AUTO, TSQR_LEAF_GRAM_TREE,
+    from tsqr_amd._lib import (TSQR_DOUBLE, TSQR_FLOAT, TSQR_LEAF_AUTO, TSQR_LEAF_CHOLQR2, TSQR_LEAF_GRAM_TREE,
                                TSQR_LEAF_HOUSEHOLDER, leaf_stats, mixed_stats, set_leaf_path, timing_read)
 
     kind, m, n, dtype = WORKLOADS[args.workload]
@@ -240,8 +240,9 @@ def main():
     fn = getattr(L, f"{kind}_{'s' if code == TSQR_FLOAT else 'd'}_stream")
     is_tsqr = kind in ("tsqr", "tsqr_mixed")
     if args.leaf_path != "default":
-        set_leaf_path(code, {"auto": TSQR_LEAF_AUTO, "householder": TSQR_LEAF_HOUSEHOLDER,
-                             "gram_tree": TSQR_LEAF_GRAM_TREE}[args.leaf_path])
+        path = {"auto": TSQR_LEAF_AUTO, "householder": TSQR_LEAF_HOUSEHOLDER, "gram_tree": TSQR_LEAF_GRAM_TREE,
+                "cholqr2": TSQR_LEAF_CHOLQR2}[args.leaf_path]
+        set_leaf_path(code, path, 1e5 if (path == TSQR_LEAF_CHOLQR2 and code == TSQR_DOUBLE) else 1024.0)
     stream = torch.cuda.current_stream(dev)
 
     def step():

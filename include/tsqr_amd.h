@@ -61,6 +61,7 @@ extern "C" {
 #define TSQR_LEAF_AUTO 0        /* flat Gram path if the whole matrix passes the gate, else GRAM_TREE */
 #define TSQR_LEAF_HOUSEHOLDER 1 /* Householder leaves and tree only */
 #define TSQR_LEAF_GRAM_TREE 2   /* Gram leaves (Householder where rejected), Householder tree */
+#define TSQR_LEAF_CHOLQR2 3     /* flat CholeskyQR + residual check + optional second pass, else tree */
 
 /* Human-readable text for a status code. */
 const char *tsqr_status_string(int status);
@@ -173,11 +174,20 @@ int tsqr_batch_destroy(tsqr_batch *plan);
  *   TSQR_LEAF_AUTO         as GRAM_TREE, except that when the sum of all
  *                          superleaf Grams itself passes the gate, R is its
  *                          Cholesky factor and Q = A R^{-1} (FP64), with no
- *                          tree (the same error bounds).
- * Defaults: FP32 TSQR_LEAF_AUTO with gate 1024, FP64 TSQR_LEAF_HOUSEHOLDER.
- * Below the gate the Gram path's backward error and loss of orthogonality
- * stay at the working roundoff (FP64 accumulation keeps u64*kappa^2 below
- * it).  Calls already enqueued are not affected. */
+ *                          tree (the same error bounds);
+ *   TSQR_LEAF_CHOLQR2      the whole matrix: R1 = chol(A^T A), Q1 = A R1^{-1}
+ *                          with ||Q1^T Q1 - I||_F measured in the same pass;
+ *                          above the tolerance a second CholeskyQR pass runs
+ *                          (CholeskyQR2); if the Cholesky breaks down or
+ *                          ||R||_F ||R^{-1}||_F exceeds the gate, the
+ *                          Householder tree.  FP32: the BF16x3 Gram of the
+ *                          mixed entry points, tolerance 8e-7; FP64: FP64
+ *                          Gram, tolerance 4*32*u64.
+ * Defaults: FP32 TSQR_LEAF_AUTO with gate 1024, FP64 TSQR_LEAF_CHOLQR2 with
+ * gate 1e5 (u64*kappa^2 small enough for the second pass to restore
+ * orthogonality).  Below the gate the Gram paths' backward error and loss of
+ * orthogonality stay at the working roundoff.  Calls already enqueued are not
+ * affected. */
 int tsqr_set_leaf_path(int dtype, int path, double gate);
 
 /* After a tsqr_*_stream call that used `work` has been enqueued: waits for

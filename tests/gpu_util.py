@@ -35,17 +35,20 @@ def from_device_colmajor(buf, m: int, n: int, lda: int, offset_elems: int = 0) -
 class LeafPath:
     """Context manager selecting the level-0 leaf path for one dtype."""
 
-    def __init__(self, dtype, path: int, gate: float = 1024.0):
+    def __init__(self, dtype, path: int, gate: float | None = None):
         from tsqr_amd._lib import TSQR_DOUBLE, TSQR_FLOAT
 
         self.code = TSQR_FLOAT if np.dtype(dtype) == np.float32 else TSQR_DOUBLE
+        if gate is None:  # the gate each path is meant for
+            gate = 1e5 if (self.code == TSQR_DOUBLE and path == 3) else 1024.0
         self.path, self.gate = path, gate
 
     def __enter__(self):
-        from tsqr_amd._lib import TSQR_LEAF_AUTO, TSQR_LEAF_HOUSEHOLDER, TSQR_FLOAT, set_leaf_path
+        from tsqr_amd._lib import TSQR_LEAF_AUTO, TSQR_LEAF_CHOLQR2, TSQR_FLOAT, set_leaf_path
 
         set_leaf_path(self.code, self.path, self.gate)
-        self.default = (TSQR_LEAF_AUTO if self.code == TSQR_FLOAT else TSQR_LEAF_HOUSEHOLDER, 1024.0)
+        # the library's defaults: FP32 AUTO (gate 1024), FP64 CHOLQR2 (gate 1e5)
+        self.default = (TSQR_LEAF_AUTO, 1024.0) if self.code == TSQR_FLOAT else (TSQR_LEAF_CHOLQR2, 1e5)
         return self
 
     def __exit__(self, *exc):

@@ -19,9 +19,9 @@ from tests.gpu_util import LAST_LEAF_STATS, LeafPath, call
 
 pytestmark = pytest.mark.gpu
 
-AUTO, HOUSEHOLDER, GRAM_TREE = 0, 1, 2
+AUTO, HOUSEHOLDER, GRAM_TREE, CHOLQR2 = 0, 1, 2, 3
 PATHS = [pytest.param(AUTO, id="auto"), pytest.param(HOUSEHOLDER, id="householder"),
-         pytest.param(GRAM_TREE, id="gram_tree")]
+         pytest.param(GRAM_TREE, id="gram_tree"), pytest.param(CHOLQR2, id="cholqr2")]
 
 
 def check(A, Q, R, tag, c=8.0, n_floor=1):

@@ -13,6 +13,7 @@ TSQR_DOUBLE = 2
 TSQR_LEAF_AUTO = 0
 TSQR_LEAF_HOUSEHOLDER = 1
 TSQR_LEAF_GRAM_TREE = 2
+TSQR_LEAF_CHOLQR2 = 3
 TSQR_BATCH_PLAIN = 0
 TSQR_BATCH_MIXED = 1
 

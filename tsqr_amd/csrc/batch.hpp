@@ -356,8 +356,8 @@ __global__ __launch_bounds__(64, 2) void batch_mixed_q(const BatchProb* __restri
     const int64_t T0 = L0.tiles;
     dispatch_nv(p.N, p.vec, [&](auto NC, auto VC) {
         constexpr int NT = decltype(NC)::value > 16 ? 2 : 1;
-        mixed_q_body<NT, decltype(VC)::value>(L0, p.d.n, reinterpret_cast<float*>(p.A), p.lda, side + T0 * 1024,
-                                              side + (int64_t)it.y * 1024, sums, it.y);
+        mixed_q_body<float, NT, decltype(VC)::value>(L0, p.d.n, reinterpret_cast<float*>(p.A), p.lda,
+                                                     side + T0 * 1024, side + (int64_t)it.y * 1024, sums, it.y);
     });
 }
 
@@ -376,8 +376,8 @@ __global__ __launch_bounds__(64) void batch_mixed_check(const BatchProb* __restr
     double* side = prob_side(p, work);
     const int64_t T0 = p.d.lv[0].tiles;
     sum_blocks_wave(L, side, (int)T0, lane);
-    mixed_check_body(L, p.d.n, reinterpret_cast<float*>(p.R), p.ldr, side + T0 * 1024, side + (T0 + 1) * 1024,
-                     side + (T0 + 2) * 1024, ctl, pass, tol, lane);
+    mixed_check_body<float>(L, p.d.n, reinterpret_cast<float*>(p.R), p.ldr, side + T0 * 1024,
+                            side + (T0 + 1) * 1024, side + (T0 + 2) * 1024, ctl, pass, tol, lane);
 }
 
 // ---------------------------------------------------------------------------

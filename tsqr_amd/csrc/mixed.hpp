@@ -202,17 +202,21 @@ __global__ __launch_bounds__(64) void tsqr_mixed_gram(const TreeDesc d, const fl
 // Pass 1b / 2: Q = A X in place for every superleaf (X upper triangular,
 // FP64 row-major 32 x 32 in side block T0), products and sums in FP64 on the
 // matrix cores and one rounding, as in tsqr_gram_q's flat case; then the
-// Gram matrix of the rounded rows (BF16x3) goes to side[leaf].  Pass 1 runs
-// when the flat Gram path was taken (ctl[0]); pass 2 only if the check asked
-// for it (ctl[1]).
+// Gram matrix of the rounded rows goes to side[leaf]: BF16x3 for float, FP64
+// MFMA for double.  Pass 1 runs when the flat Gram path was taken (ctl[0]);
+// pass 2 only if the check asked for it (ctl[1]).
 //
 // Layout: rows in blocks of 64; the FP64 MFMA output register v of (t, jt) is
-// row r0 + 16v + 4(l/16) + t, column 16 jt + l%16.  For the Gram, the 32
-// rows r0 + 32h + [0, 32) feed k-slots s = 4(v - 2h) + t of lane l, i.e.
-// k = 8(l/16) + s <-> row 16(v - 2h) + 4(l/16) + t: a bijection onto the 32
-// rows, identical for both operands.
-template <int NT, bool VEC>
-__device__ __forceinline__ void mixed_q_body(const LevelDesc& L0, int n, float* __restrict__ A, int lda,
+// row r0 + 16v + 4(l/16) + t, column 16 jt + l%16.
+//   float:  the 32 rows r0 + 32h + [0, 32) feed BF16 k-slots
+//           s = 4(v - 2h) + t of lane l, i.e. k = 8(l/16) + s <-> row
+//           16(v - 2h) + 4(l/16) + t: a bijection onto the 32 rows,
+//           identical for both operands;
+//   double: register (v, t) of lane l is the 16x16x4 f64 operand with
+//           k = l/16 <-> row 16v + 4(l/16) + t, again the same for both
+//           operands.
+template <typename T, int NT, bool VEC>
+__device__ __forceinline__ void mixed_q_body(const LevelDesc& L0, int n, T* __restrict__ A, int lda,
                                              const double* __restrict__ X, double* __restrict__ Gout, double* sums,
                                              int leaf) {
     const int lane = threadIdx.x & 63;
@@ -224,17 +228,20 @@ __device__ __forceinline__ void mixed_q_body(const LevelDesc& L0, int n, float* 
 #pragma unroll
         for (int jt = 0; jt < 2; ++jt) mb[kg][jt] = X[(4 * kg + lq) * 32 + 16 * jt + li];
     const int rows = leaf_rows_of(L0, leaf);
-    float* base = A + (int64_t)leaf * L0.tile_rows;
+    T* base = A + (int64_t)leaf * L0.tile_rows;
     const int nk = (n + 3) / 4;
-    Raw4<float> raw[8];
+    Raw4<T> raw[8];
 #pragma unroll
     for (int kg = 0; kg < 8; ++kg) {
         const int k = 4 * kg + lq;
-        if (kg < nk) raw[kg] = load_raw4<float, VEC>(base, lda, 4 * li, k < n ? k : n - 1, rows, n);
+        if (kg < nk) raw[kg] = load_raw4<T, VEC>(base, lda, 4 * li, k < n ? k : n - 1, rows, n);
     }
-    GramAcc32 g;
-    g.sums = sums;
-    g.zero(lane);
+    GramAcc32 g;  // float: BF16x3, FP64 sums in LDS
+    v4d d00 = {0, 0, 0, 0}, d01 = {0, 0, 0, 0}, d11 = {0, 0, 0, 0};  // double: FP64 MFMA
+    if constexpr (sizeof(T) == 4) {
+        g.sums = sums;
+        g.zero(lane);
+    }
 #pragma unroll 1
     for (int r0 = 0; r0 < rows; r0 += 64) {
         double a[8][4];
@@ -247,7 +254,7 @@ __device__ __forceinline__ void mixed_q_body(const LevelDesc& L0, int n, float* 
 #pragma unroll
             for (int kg = 0; kg < 8; ++kg) {
                 const int k = 4 * kg + lq;
-                if (kg < nk) raw[kg] = load_raw4<float, VEC>(base, lda, r1 + 4 * li, k < n ? k : n - 1, rows, n);
+                if (kg < nk) raw[kg] = load_raw4<T, VEC>(base, lda, r1 + 4 * li, k < n ? k : n - 1, rows, n);
             }
         }
         v4d acc[4][2];
@@ -268,43 +275,68 @@ __device__ __forceinline__ void mixed_q_body(const LevelDesc& L0, int n, float* 
         }
         // q[jt][v][t] = Q[r0 + 16v + 4 lq + t][16 jt + li], rounded; zero in
         // padded columns (rows past the end are zero already)
-        float q[2][4][4];
+        T q[2][4][4];
 #pragma unroll
         for (int jt = 0; jt < 2; ++jt) {
             const int cc = 16 * jt + li;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
 #pragma unroll
-                for (int t = 0; t < 4; ++t) q[jt][v][t] = cc < n ? (float)acc[t][jt][v] : 0.0f;
+                for (int t = 0; t < 4; ++t) q[jt][v][t] = cc < n ? (T)acc[t][jt][v] : T(0);
                 if (cc < n)
-                    store_column<float, 4, VEC>(q[jt][v], base + (int64_t)cc * lda + r0 + 16 * v + 4 * lq, rows,
-                                                r0 + 16 * v + 4 * lq);
+                    store_column<T, 4, VEC>(q[jt][v], base + (int64_t)cc * lda + r0 + 16 * v + 4 * lq, rows,
+                                            r0 + 16 * v + 4 * lq);
             }
         }
+        if constexpr (sizeof(T) == 4) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            float x0[8], x1[8];
+            for (int h = 0; h < 2; ++h) {
+                float x0[8], x1[8];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                x0[t] = q[0][2 * h][t];
-                x0[4 + t] = q[0][2 * h + 1][t];
-                x1[t] = q[1][2 * h][t];
-                x1[4 + t] = q[1][2 * h + 1][t];
+                for (int t = 0; t < 4; ++t) {
+                    x0[t] = q[0][2 * h][t];
+                    x0[4 + t] = q[0][2 * h + 1][t];
+                    x1[t] = q[1][2 * h][t];
+                    x1[4 + t] = q[1][2 * h + 1][t];
+                }
+                g.add<NT>(x0, x1);
             }
-            g.add<NT>(x0, x1);
+            g.flush<NT>(lane);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    d00 = __builtin_amdgcn_mfma_f64_16x16x4f64(q[0][v][t], q[0][v][t], d00, 0, 0, 0);
+                    if constexpr (NT == 2) {
+                        d01 = __builtin_amdgcn_mfma_f64_16x16x4f64(q[0][v][t], q[1][v][t], d01, 0, 0, 0);
+                        d11 = __builtin_amdgcn_mfma_f64_16x16x4f64(q[1][v][t], q[1][v][t], d11, 0, 0, 0);
+                    }
+                }
         }
-        g.flush<NT>(lane);
     }
-    g.store(Gout, lane);
+    if constexpr (sizeof(T) == 4) {
+        g.store(Gout, lane);
+    } else {
+        // f64 accumulator: lane l, register v holds C[4v + l/16][l%16]
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 4 * v + lq;
+            Gout[i * 32 + li] = d00[v];
+            Gout[i * 32 + 16 + li] = d01[v];
+            Gout[(16 + li) * 32 + i] = d01[v];
+            Gout[(16 + i) * 32 + 16 + li] = d11[v];
+        }
+    }
 }
 
-template <int NT, bool VEC>
-__global__ __launch_bounds__(64) void tsqr_mixed_q(const TreeDesc d, float* __restrict__ A, int lda,
+template <typename T, int NT, bool VEC>
+__global__ __launch_bounds__(64) void tsqr_mixed_q(const TreeDesc d, T* __restrict__ A, int lda,
                                                    double* __restrict__ side, const int* __restrict__ ctl, int pass) {
     __shared__ double sums[12 * 64];
     if (!ctl[0] || (pass == 2 && !ctl[1])) return;
-    mixed_q_body<NT, VEC>(d.lv[0], d.n, A, lda, side + (int64_t)d.lv[0].tiles * 1024,
-                          side + (int64_t)blockIdx.x * 1024, sums, blockIdx.x);
+    mixed_q_body<T, NT, VEC>(d.lv[0], d.n, A, lda, side + (int64_t)d.lv[0].tiles * 1024,
+                             side + (int64_t)blockIdx.x * 1024, sums, blockIdx.x);
 }
 
 // Residual check after pass `pass`, by one wave (lanes 0..63), on the Gram
@@ -313,7 +345,8 @@ __global__ __launch_bounds__(64) void tsqr_mixed_q(const TreeDesc d, float* __re
 // otherwise R2 = chol(G), R = R2 R1 (R1 in FP64, row-major) to the caller,
 // X = R2^{-1} (FP64, row-major) and ctl[1] = 1.  Pass 2: ctl[2] = (e > tol).
 // A second Cholesky that breaks down sets ctl[2] = 2 and keeps Q1, R1.
-__device__ __forceinline__ void mixed_check_body(GramLds& L, int n, float* __restrict__ R, int ldr,
+template <typename T>
+__device__ __forceinline__ void mixed_check_body(GramLds& L, int n, T* __restrict__ R, int ldr,
                                                  double* __restrict__ X, const double* __restrict__ R1,
                                                  double* __restrict__ misc, int* __restrict__ ctl, int pass,
                                                  double tol, int lane) {
@@ -355,7 +388,7 @@ __device__ __forceinline__ void mixed_check_body(GramLds& L, int n, float* __res
             double s = 0.0;
 #pragma unroll 1
             for (int k = r; k <= c; ++k) s = fmat(L.G[r][k], R1[k * 32 + c], s);
-            R[r + (int64_t)c * ldr] = (r <= c) ? (float)s : 0.0f;
+            R[r + (int64_t)c * ldr] = (r <= c) ? (T)s : T(0);
         }
     }
     double x[32];
@@ -370,10 +403,9 @@ __device__ __forceinline__ void mixed_check_body(GramLds& L, int n, float* __res
 // Check kernel (one workgroup of 256 threads): G = sum of the nparts
 // partials (in order), then mixed_check_body with X = side block T0, R1 in
 // side block T0+1 and the residuals in side block T0+2.
-// (A template only so that every translation unit gets its own copy.)
-template <int UNUSED = 0>
+template <typename T>
 __global__ __launch_bounds__(256) void tsqr_mixed_check(const TreeDesc d, int nparts,
-                                                        const double* __restrict__ partial, float* __restrict__ R,
+                                                        const double* __restrict__ partial, T* __restrict__ R,
                                                         int ldr, double* __restrict__ side, int* __restrict__ ctl,
                                                         int pass, double tol) {
     __shared__ GramLds L;
@@ -383,8 +415,8 @@ __global__ __launch_bounds__(256) void tsqr_mixed_check(const TreeDesc d, int np
     __syncthreads();
     if (t >= 64) return;
     const int64_t T0 = d.lv[0].tiles;
-    mixed_check_body(L, d.n, R, ldr, side + T0 * 1024, side + (T0 + 1) * 1024, side + (T0 + 2) * 1024, ctl, pass,
-                     tol, t);
+    mixed_check_body<T>(L, d.n, R, ldr, side + T0 * 1024, side + (T0 + 1) * 1024, side + (T0 + 2) * 1024, ctl,
+                        pass, tol, t);
 }
 
 }  // namespace tsqr

@@ -99,11 +99,15 @@ size_t tsqr_ws_bytes(int m, int n) {
 // with the Householder fallback; FP64 has no wider type to accumulate the
 // Gram matrix in, so it defaults to Householder leaves.
 // The mixed entry points have their own gate and residual tolerance
-// (tsqr_set_mixed).
+// (tsqr_set_mixed).  FP64 defaults to CholeskyQR2 with an FP64 Gram: the
+// gate keeps u64 kappa^2 small enough for the second pass to restore
+// orthogonality, and the residual check asks for 4 n u64 with n = 32.
+constexpr double kTolF64 = 4.0 * 32.0 * 1.1102230246251565e-16;
+constexpr double kGateF64 = 1e5;
 std::mutex g_opts_mu;
 LaunchOpts g_opts_f32{kLeafAuto, 1024.0, 0.0};
-LaunchOpts g_opts_f64{kLeafHouseholder, 1024.0, 0.0};
-LaunchOpts g_opts_mixed{kLeafMixed, 1024.0, 8e-7};
+LaunchOpts g_opts_f64{kLeafCholQR2, kGateF64, kTolF64};
+LaunchOpts g_opts_mixed{kLeafCholQR2, 1024.0, 8e-7};
 
 template <typename T>
 LaunchOpts current_opts() {
@@ -297,13 +301,15 @@ int tsqr_timing_read(double* ms, int* count, int nphases) {
 }
 
 int tsqr_set_leaf_path(int dtype, int path, double gate) {
-    if ((path != TSQR_LEAF_AUTO && path != TSQR_LEAF_HOUSEHOLDER && path != TSQR_LEAF_GRAM_TREE) || !(gate >= 1.0))
+    if ((path != TSQR_LEAF_AUTO && path != TSQR_LEAF_HOUSEHOLDER && path != TSQR_LEAF_GRAM_TREE &&
+         path != TSQR_LEAF_CHOLQR2) ||
+        !(gate >= 1.0))
         return TSQR_ERR_ARG;
     std::lock_guard<std::mutex> lock(g_opts_mu);
     if (dtype == TSQR_FLOAT)
-        g_opts_f32 = LaunchOpts{path, gate, 0.0};
+        g_opts_f32 = LaunchOpts{path, gate, g_opts_mixed.tol};
     else if (dtype == TSQR_DOUBLE)
-        g_opts_f64 = LaunchOpts{path, gate, 0.0};
+        g_opts_f64 = LaunchOpts{path, gate, kTolF64};
     else
         return TSQR_ERR_ARG;
     return TSQR_OK;
@@ -334,6 +340,10 @@ int tsqr_leaf_stats(const void* work, int m, int n, int dtype, int* gram_leaves,
     for (int f : flags) (f ? *householder_leaves : *gram_leaves) += 1;
     if (hipMemcpy(flat, reinterpret_cast<const char*>(work) + w.ctl, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
         return TSQR_ERR_HIP;
+    if (*flat) {  // the whole matrix took a flat Gram path (AUTO or CHOLQR2)
+        *gram_leaves = d.lv[0].tiles;
+        *householder_leaves = 0;
+    }
     return TSQR_OK;
 }
 

@@ -55,8 +55,9 @@ enum LeafPath {
     kLeafAuto = 0,         // flat Gram path if the whole matrix passes the gate, else kLeafGramTree
     kLeafHouseholder = 1,  // Householder for every superleaf
     kLeafGramTree = 2,     // Gram path per superleaf (Householder where rejected) + Householder tree
-    kLeafMixed = 3,        // FP32 only: BF16x3 CholeskyQR with residual check and optional
-                           // second pass (mixed.hpp), Householder tree if the gate rejects
+    kLeafCholQR2 = 3,      // flat CholeskyQR with a residual check and an optional second pass
+                           // (mixed.hpp): BF16x3 Gram for FP32 (the mixed-precision method),
+                           // FP64 MFMA Gram for FP64; Householder tree if the gate rejects
 };
 
 struct LaunchOpts {

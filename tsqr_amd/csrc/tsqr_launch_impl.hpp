@@ -15,10 +15,11 @@
 //   down          levels root-1 .. 1, one launch each
 //   gram_q        M = R^{-1} S, then Q = A M, for the Gram superleaves
 //   super_q       Householder Q for the flagged superleaves
-// The mixed path (FP32) replaces the Gram kernels by mixed_gram, the flat
-// reduction, mixed_q + mixed_check (pass 1) and mixed_q + mixed_check
-// (pass 2, a no-op unless the check asks for it); every superleaf is flagged
-// Householder so that the tree path takes over if the flat gate rejects.
+// The CholeskyQR2 path replaces the Gram kernels by mixed_gram (FP32,
+// BF16x3) or gram_g (FP64), the flat reduction and Cholesky, mixed_q +
+// mixed_check (pass 1) and mixed_q + mixed_check (pass 2, a no-op unless
+// the check asks for it); every superleaf is flagged Householder so that the
+// tree path takes over if the flat gate rejects.
 // A single tile (m <= H) is one tsqr_single launch.
 #pragma once
 #include "../../include/tsqr_amd.h"
@@ -86,13 +87,10 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     double* partial = reinterpret_cast<double*>(base + w.partial);
     double* side = reinterpret_cast<double*>(base + w.side);
     const int leaves = d.lv[0].tiles;
-    const bool mixed = o.path == kLeafMixed;
+    const bool mixed = o.path == kLeafCholQR2;
     const bool gram = o.path == kLeafAuto || o.path == kLeafGramTree;
     const int force = gram ? 0 : 1;
     constexpr int NT = N > 16 ? 2 : 1;
-    if constexpr (sizeof(T) != 4) {
-        if (mixed) return TSQR_ERR_ARG;
-    }
 
     const int ninit = d.total_ctr > leaves ? d.total_ctr : leaves;
     hipLaunchKernelGGL(init_flags, dim3((ninit + 63 + 255) / 256), dim3(256), 0, st, tickets, d.total_ctr, flags,
@@ -101,15 +99,16 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     const int super_grid = leaves < 1024 ? leaves : 1024;
     double* rkeep = side + (size_t)(leaves + 1) * 1024;
     if (timed) timing_mark(kPhaseLeafR, true, st);
-    if constexpr (sizeof(T) == 4) {
-        if (mixed) {
+    if (mixed) {
+        if constexpr (sizeof(T) == 4)
             hipLaunchKernelGGL((tsqr_mixed_gram<NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const float*)A, lda,
                                side);
-            int np = 0;
-            const double* parts = reduce_grams(leaves, side, partial, nullptr, &np, st);
-            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, np, parts, R, ldr, side, ctl,
-                               o.gate, rkeep);
-        }
+        else
+            hipLaunchKernelGGL((tsqr_gram_g<T, NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const T*)A, lda, side);
+        int np = 0;
+        const double* parts = reduce_grams(leaves, side, partial, nullptr, &np, st);
+        hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, np, parts, R, ldr, side, ctl, o.gate,
+                           rkeep);
     }
     if (gram) {
         hipLaunchKernelGGL((tsqr_gram_g<T, NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const T*)A, lda, side);
@@ -137,16 +136,14 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
                            (const int*)ctl);
     if (timed) timing_mark(kPhaseDown, false, st);
     if (timed) timing_mark(kPhaseLeafQ, true, st);
-    if constexpr (sizeof(T) == 4) {
-        if (mixed) {
-            for (int pass = 1; pass <= 2; ++pass) {
-                hipLaunchKernelGGL((tsqr_mixed_q<NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (float*)A, lda, side,
-                                   (const int*)ctl, pass);
-                int np = 0;
-                const double* parts = reduce_grams(leaves, side, partial, ctl + (pass == 1 ? 0 : 1), &np, st);
-                hipLaunchKernelGGL(tsqr_mixed_check<>, dim3(1), dim3(256), 0, st, d, np, parts, (float*)R, ldr, side,
-                                   ctl, pass, o.tol);
-            }
+    if (mixed) {
+        for (int pass = 1; pass <= 2; ++pass) {
+            hipLaunchKernelGGL((tsqr_mixed_q<T, NT, VEC>), dim3(leaves), dim3(64), 0, st, d, A, lda, side,
+                               (const int*)ctl, pass);
+            int np = 0;
+            const double* parts = reduce_grams(leaves, side, partial, ctl + (pass == 1 ? 0 : 1), &np, st);
+            hipLaunchKernelGGL(tsqr_mixed_check<T>, dim3(1), dim3(256), 0, st, d, np, parts, R, ldr, side, ctl, pass,
+                               o.tol);
         }
     }
     if (gram)
