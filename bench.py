@@ -37,6 +37,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -78,7 +80,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="tsqr_max_fp32", choices=sorted(WORKLOADS) + ["batch_512_fp32"])
+    p.add_argument("--workload", default="tsqr_max_fp32",
+                   choices=sorted(WORKLOADS) + ["batch_512_fp32", "qr_mixed_1m_sweep"])
     p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     p.add_argument("--cpu-fraction", type=int, default=8, help="cpu sample = m / this")
     p.add_argument("--copy-bytes", type=int, default=0,
@@ -90,6 +93,59 @@ def parse():
     p.add_argument("--batch-steps", type=int, default=5)
     p.add_argument("--batch-count", type=int, default=512)
     return p.parse_args()
+
+
+def cond_sweep(steps: int, m: int = 1048576) -> dict:
+    """Mixed-precision recursive QR vs the plain qr_s at m = 1,048,576,
+    n in {64, 128}, A = U diag(s) V^T (U, V from FP64 QR of seeded Gaussians
+    on the GPU, s geometric from 1 to 1/cond), cond in {1e2, 1e4, 1e6}.
+    Each step restores A (untimed) and times one call with HIP events on the
+    launch stream; errors of the last call in FP64 on the GPU against the
+    FP64 A (before its rounding to FP32) and the FP32 A."""
+    import torch
+
+    import tsqr_amd
+
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    g = torch.Generator(device=dev)
+    out = {"m": m, "cases": []}
+    for n in (64, 128):
+        for cond in (1e2, 1e4, 1e6):
+            g.manual_seed(n * 7 + int(np.log10(cond)))
+            U, _ = torch.linalg.qr(torch.randn(m, n, dtype=torch.float64, device=dev, generator=g))
+            V, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+            sv = cond ** (-torch.arange(n, dtype=torch.float64, device=dev) / (n - 1))
+            A64 = (U * sv) @ V.T
+            del U
+            src = tsqr_amd.colmajor(A64.float())
+            A32 = src.double()
+            case = {"n": n, "cond": cond}
+            for name, fn in (("mixed", tsqr_amd.qr_mixed), ("plain", tsqr_amd.qr)):
+                A = src.clone(memory_format=torch.preserve_format)
+                R = tsqr_amd.empty_colmajor(n, n, torch.float32, dev)
+                work = torch.empty(tsqr_amd.qr_workspace_bytes(m, n, 1), dtype=torch.uint8, device=dev)
+                times = []
+                for k in range(steps + 1):
+                    A.copy_(src)
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    fn(A, R, work, stream)
+                    e1.record(stream)
+                    torch.cuda.synchronize()
+                    if k:
+                        times.append(e0.elapsed_time(e1))
+                Q64, R64 = A.double(), torch.triu(R.double())
+                back = float(torch.linalg.norm(A32 - Q64 @ R64) / torch.linalg.norm(A32))
+                orth = float(torch.linalg.norm(torch.eye(n, dtype=torch.float64, device=dev) - Q64.T @ Q64))
+                case[name] = {"ms": round(sum(times) / len(times), 4), "backward": float(f"{back:.3e}"),
+                              "orth": float(f"{orth:.3e}")}
+                del A, work, Q64
+            print(f"# sweep {case}", file=sys.stderr, flush=True)
+            out["cases"].append(case)
+            del A64, src, A32
+            torch.cuda.empty_cache()
+    return out
 
 
 def batched_leg(steps: int, count: int) -> dict:
@@ -208,6 +264,17 @@ def main():
 
     import tsqr_amd
 
+    if args.workload == "qr_mixed_1m_sweep":
+        torch.cuda.set_device(0)
+        sw = cond_sweep(args.steps)
+        worst = max(c["mixed"]["orth"] for c in sw["cases"])
+        print(json.dumps({"metric": "mixed-precision recursive QR: max ||I-Q^TQ||_F over cond in {1e2,1e4,1e6}",
+                          "value": worst, "unit": "1", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+                          "ms_per_step": max(c["mixed"]["ms"] for c in sw["cases"]), "higher_is_better": False,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                          "data": "synthetic: U diag(s) V^T from seeded Gaussians",
+                          "config": {"workload": "qr_mixed m=1048576 n in {64,128}"}, "sweep": sw}), flush=True)
+        return
     if args.workload == "batch_512_fp32":
         # the batched leg alone: value = mixed-method QRs/s
         torch.cuda.set_device(0)

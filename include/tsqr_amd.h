@@ -110,6 +110,18 @@ int tsqr_mixed_s(int m, int n, float *A, int lda, float *R, int ldr, float *work
 int tsqr_mixed_s_stream(int m, int n, float *A, int lda, float *R, int ldr, void *work, size_t work_bytes,
                         void *stream);
 
+/* Mixed-precision recursive blocked QR (FP32 in and out), the qr<float>
+ * contract (block_QR.h:72-95) with a different method; no counterpart in the
+ * reference.  The same column split as qr_s; every panel of <= 32 columns is
+ * tsqr_mixed_s, and the coupling of the column blocks is applied twice
+ * (block classical Gram-Schmidt with reorthogonalisation): R12 = Q1^T A2,
+ * A2 -= Q1 R12, then C = Q1^T A2, A2 -= Q1 C, R12 += C (FP32 matrix-core
+ * products), which keeps the blocks orthogonal to each other independently of
+ * cond(A).  Workspace: qr_workspace_bytes(m, n, TSQR_FLOAT). */
+int qr_mixed_s(int m, int n, float *A, int lda, float *R, int ldr, float *work, int ldwork);
+int qr_mixed_s_stream(int m, int n, float *A, int lda, float *R, int ldr, void *work, size_t work_bytes,
+                      void *stream);
+
 /* Gate (>= 1, default 1024) and residual tolerance (> 0, default 8e-7) of
  * the mixed entry points, process-wide. */
 int tsqr_set_mixed(double gate, double tol);

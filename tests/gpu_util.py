@@ -78,7 +78,7 @@ def call(kind: str, A: np.ndarray, lda: int | None = None, offset_elems: int = 0
     suf = "s" if code == TSQR_FLOAT else "d"
     buf, pA, lda = to_device_colmajor(A, lda, offset_elems)
     Rbuf = torch.full((n * n,), float("nan"), dtype=torch_dtype(A.dtype), device="cuda")
-    need = (L.qr_workspace_bytes if kind == "qr" else L.tsqr_workspace_bytes)(m, n, code)
+    need = (L.qr_workspace_bytes if kind.startswith("qr") else L.tsqr_workspace_bytes)(m, n, code)
     if ref_shaped:
         elem = 4 if code == TSQR_FLOAT else 8
         work = torch.empty(max(ldwork * n * elem, 1), dtype=torch.uint8, device="cuda")
@@ -86,7 +86,8 @@ def call(kind: str, A: np.ndarray, lda: int | None = None, offset_elems: int = 0
     else:
         nbytes = need if work_bytes is None else work_bytes
         work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device="cuda")
-        st = getattr(L, f"{kind}_{suf}_stream")(m, n, pA, lda, Rbuf.data_ptr(), n, work.data_ptr(), nbytes, None)
+        st = getattr(L, f"{kind}_{'s' if 'mixed' in kind else suf}_stream")(m, n, pA, lda, Rbuf.data_ptr(), n,
+                                                                            work.data_ptr(), nbytes, None)
         if st == 0 and kind == "tsqr":
             from tsqr_amd._lib import leaf_stats
 

@@ -31,7 +31,7 @@ from ._lib import (  # noqa: F401
     qr_workspace_bytes,
     tsqr_workspace_bytes,
 )
-from .api import Batch, colmajor, empty_colmajor, qr, tsqr, tsqr_mixed  # noqa: F401
+from .api import Batch, colmajor, empty_colmajor, qr, qr_mixed, tsqr, tsqr_mixed  # noqa: F401
 
 __all__ = [
     "Batch",
@@ -41,6 +41,7 @@ __all__ = [
     "empty_colmajor",
     "lib",
     "qr",
+    "qr_mixed",
     "qr_workspace_bytes",
     "tsqr",
     "tsqr_mixed",

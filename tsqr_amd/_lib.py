@@ -38,6 +38,8 @@ EXPORTS = (
     "tsqr_mixed_s_stream",
     "tsqr_set_mixed",
     "tsqr_mixed_stats",
+    "qr_mixed_s",
+    "qr_mixed_s_stream",
     "tsqr_batch_create",
     "tsqr_batch_workspace_bytes",
     "tsqr_batch_run",
@@ -74,11 +76,12 @@ def lib() -> ctypes.CDLL:
     for f in (L.tsqr_workspace_bytes, L.qr_workspace_bytes):
         f.argtypes = [i32, i32, i32]
         f.restype = sz
-    for name in ("tsqr_s", "tsqr_d", "qr_s", "qr_d", "tsqr_mixed_s"):
+    for name in ("tsqr_s", "tsqr_d", "qr_s", "qr_d", "tsqr_mixed_s", "qr_mixed_s"):
         f = getattr(L, name)
         f.argtypes = [i32, i32, vp, i32, vp, i32, vp, i32]
         f.restype = i32
-    for name in ("tsqr_s_stream", "tsqr_d_stream", "qr_s_stream", "qr_d_stream", "tsqr_mixed_s_stream"):
+    for name in ("tsqr_s_stream", "tsqr_d_stream", "qr_s_stream", "qr_d_stream", "tsqr_mixed_s_stream",
+                 "qr_mixed_s_stream"):
         f = getattr(L, name)
         f.argtypes = [i32, i32, vp, i32, vp, i32, vp, sz, vp]
         f.restype = i32

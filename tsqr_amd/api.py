@@ -36,14 +36,14 @@ def _run(kind: str, A: torch.Tensor, R, work, stream):
     if R is None:
         R = empty_colmajor(n, n, A.dtype, A.device)
     ldr = _ld(R, "R")
-    need = (qr_workspace_bytes if kind == "qr" else tsqr_workspace_bytes)(m, n, code)
+    need = (qr_workspace_bytes if kind.startswith("qr") else tsqr_workspace_bytes)(m, n, code)
     if work is None:
         work = torch.empty(max(need, 1), dtype=torch.uint8, device=A.device)
     nbytes = work.numel() * work.element_size()
     if stream is None:
         stream = torch.cuda.current_stream(A.device)
     suf = "s" if code == TSQR_FLOAT else "d"
-    if kind == "tsqr_mixed":
+    if kind in ("tsqr_mixed", "qr_mixed"):
         if code != TSQR_FLOAT:
             raise TypeError("the mixed-precision TSQR takes float32 input")
         suf = "s"
@@ -70,6 +70,13 @@ def tsqr_mixed(A: torch.Tensor, R: torch.Tensor | None = None, work: torch.Tenso
     BF16x3 CholeskyQR with an FP32 residual check and an optional second pass.
     Same conventions as tsqr."""
     return _run("tsqr_mixed", A, R, work, stream)
+
+
+def qr_mixed(A: torch.Tensor, R: torch.Tensor | None = None, work: torch.Tensor | None = None, stream=None):
+    """Mixed-precision recursive blocked QR of a float32 column-major GPU
+    tensor: mixed TSQR panels and a reorthogonalised coupling.  Same
+    conventions as qr."""
+    return _run("qr_mixed", A, R, work, stream)
 
 
 class Batch:

@@ -155,16 +155,18 @@ __global__ __launch_bounds__(64) void gemm_tn_mfma(int m, int n1, int n2, const 
             }
 }
 
-// C[i + j*ldc] = sum over chunks of part[k][i + j*n1], chunks in order.
+// C[i + j*ldc] = (accumulate ? C : 0) + sum over chunks of
+// part[k][i + j*n1], chunks in order.
 template <typename T>
 __global__ void gemm_tn_reduce(int n1, int n2, int chunks, const T* __restrict__ part, T* __restrict__ C,
-                               int64_t ldc) {
+                               int64_t ldc, int accumulate) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n1 * n2) return;
     T s = T(0);
     for (int k = 0; k < chunks; ++k) s += part[(int64_t)k * n1 * n2 + idx];
     const int i = idx % n1, j = idx / n1;
-    C[i + (int64_t)j * ldc] = s;
+    T* c = C + i + (int64_t)j * ldc;
+    *c = accumulate ? *c + s : s;
 }
 
 // C[rows, j0:j0+64] -= X[rows, 0:K] B[0:K, j0:j0+64] for the 64-row block
@@ -229,6 +231,15 @@ __global__ void zero_block(T* __restrict__ M, int64_t ld, int r0, int nr, int c0
     if (idx >= nr * nc) return;
     const int r = idx % nr, c = idx / nr;
     M[(r0 + r) + (int64_t)(c0 + c) * ld] = T(0);
+}
+
+// dst[r + c*ldd] += src[r + c*lds] for an nr x nc block.
+template <typename T>
+__global__ void add_block(const T* __restrict__ src, int lds, T* __restrict__ dst, int64_t ldd, int nr, int nc) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nr * nc) return;
+    const int r = idx % nr, c = idx / nr;
+    dst[r + (int64_t)c * ldd] += src[r + (int64_t)c * lds];
 }
 
 }  // namespace tsqr

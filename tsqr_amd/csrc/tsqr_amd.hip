@@ -169,63 +169,93 @@ inline size_t tn_partial_elems(int m, int n) {
     return mx > b ? mx : b;
 }
 
+// Panel TSQR workspace, then the split-K partials, then n1 x n2 elements for
+// the mixed method's second coupling product.
 template <typename T>
 size_t qr_ws_bytes(int m, int n) {
     if (n < 1 || m < n) return 0;
     if (n <= 32) return tsqr_ws_bytes<T>(m, n);
     const size_t panel = tsqr_ws_bytes<T>(m, 32);
-    return align256(panel) + align256(tn_partial_elems(m, n) * sizeof(T));
+    const int n1 = split_cols(n);
+    return align256(panel) + align256(tn_partial_elems(m, n) * sizeof(T)) +
+           align256((size_t)n1 * (n - n1) * sizeof(T));
 }
 
+// One coupling step: part = chunk products of Q1^T A2, C = their sum (or
+// C += it), A2 -= Q1 (the new sum), as the reference's two gemm_blockqr
+// calls (block_QR.h:88-93).
 template <typename T>
-int qr_rec(int m, int n, T* A, int lda, T* R, int ldr, char* ws, size_t panel_bytes, T* part, hipStream_t st) {
-    if (n <= 32) return tsqr_core<T>(m, n, A, lda, R, ldr, ws, st);
+int couple(int m, int n1, int n2, const T* A1, T* A2, int lda, T* C, int ldc, T* part, hipStream_t st) {
+    const int kc = tn_chunk_rows(m, n1, n2);
+    const int chunks = (m + kc - 1) / kc;
+    const bool timed = timing_enabled();
+    const bool vec = ((uintptr_t)A1 % 16 == 0) && (((size_t)lda * sizeof(T)) % 16 == 0);
+    if (timed) timing_mark(kPhaseGemmTN, true, st);
+    if (vec)
+        hipLaunchKernelGGL((gemm_tn_mfma<T, true>), dim3(chunks, (n1 + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m,
+                           n1, n2, A1, (int64_t)lda, (const T*)A2, (int64_t)lda, kc, part);
+    else
+        hipLaunchKernelGGL((gemm_tn_mfma<T, false>), dim3(chunks, (n1 + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m,
+                           n1, n2, A1, (int64_t)lda, (const T*)A2, (int64_t)lda, kc, part);
+    hipLaunchKernelGGL(gemm_tn_reduce<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, n1, n2, chunks,
+                       (const T*)part, C, (int64_t)ldc, 0);
+    if (timed) timing_mark(kPhaseGemmTN, false, st);
+    if (timed) timing_mark(kPhaseGemmNN, true, st);
+    if (vec)
+        hipLaunchKernelGGL((gemm_nn_mfma<T, true>), dim3((m + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m, n1, n2,
+                           A1, (int64_t)lda, (const T*)C, (int64_t)ldc, A2, (int64_t)lda);
+    else
+        hipLaunchKernelGGL((gemm_nn_mfma<T, false>), dim3((m + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m, n1, n2,
+                           A1, (int64_t)lda, (const T*)C, (int64_t)ldc, A2, (int64_t)lda);
+    if (timed) timing_mark(kPhaseGemmNN, false, st);
+    return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
+}
+
+// Recursive blocked QR (block_QR.h:72-95).  mixed (FP32): the panels are the
+// mixed-precision TSQR and the coupling is applied twice (block classical
+// Gram-Schmidt with reorthogonalisation, BCGS2): the second product
+// C2 = Q1^T A2 is subtracted again and added to R12, which restores the
+// orthogonality of the column blocks that one pass loses in proportion to
+// cond(A).  tmp holds n1 x n2 elements.
+template <typename T>
+int qr_rec(int m, int n, T* A, int lda, T* R, int ldr, char* ws, size_t panel_bytes, T* part, T* tmp,
+           hipStream_t st, bool mixed) {
+    if (n <= 32) return tsqr_core<T>(m, n, A, lda, R, ldr, ws, st, mixed);
     const int n1 = split_cols(n), n2 = n - n1;
     T* A1 = A;
     T* A2 = A + (int64_t)n1 * lda;
     T* R11 = R;
     T* R12 = R + (int64_t)n1 * ldr;
     T* R22 = R + (int64_t)n1 * ldr + n1;
-    int e = qr_rec<T>(m, n1, A1, lda, R11, ldr, ws, panel_bytes, part, st);
+    int e = qr_rec<T>(m, n1, A1, lda, R11, ldr, ws, panel_bytes, part, tmp, st, mixed);
     if (e) return e;
-    const int kc = tn_chunk_rows(m, n1, n2);
-    const int chunks = (m + kc - 1) / kc;
-    const bool timed = timing_enabled();
-    const bool vec = ((uintptr_t)A % 16 == 0) && (((size_t)lda * sizeof(T)) % 16 == 0);
-    if (timed) timing_mark(kPhaseGemmTN, true, st);
-    if (vec)
-        hipLaunchKernelGGL((gemm_tn_mfma<T, true>), dim3(chunks, (n1 + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m,
-                           n1, n2, (const T*)A1, (int64_t)lda, (const T*)A2, (int64_t)lda, kc, part);
-    else
-        hipLaunchKernelGGL((gemm_tn_mfma<T, false>), dim3(chunks, (n1 + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m,
-                           n1, n2, (const T*)A1, (int64_t)lda, (const T*)A2, (int64_t)lda, kc, part);
-    hipLaunchKernelGGL(gemm_tn_reduce<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, n1, n2, chunks, part, R12,
-                       (int64_t)ldr);
-    if (timed) timing_mark(kPhaseGemmTN, false, st);
-    if (timed) timing_mark(kPhaseGemmNN, true, st);
-    if (vec)
-        hipLaunchKernelGGL((gemm_nn_mfma<T, true>), dim3((m + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m, n1, n2,
-                           (const T*)A1, (int64_t)lda, (const T*)R12, (int64_t)ldr, A2, (int64_t)lda);
-    else
-        hipLaunchKernelGGL((gemm_nn_mfma<T, false>), dim3((m + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m, n1, n2,
-                           (const T*)A1, (int64_t)lda, (const T*)R12, (int64_t)ldr, A2, (int64_t)lda);
-    if (timed) timing_mark(kPhaseGemmNN, false, st);
+    e = couple<T>(m, n1, n2, A1, A2, lda, R12, ldr, part, st);
+    if (e) return e;
+    if (mixed) {
+        e = couple<T>(m, n1, n2, A1, A2, lda, tmp, n1, part, st);
+        if (e) return e;
+        hipLaunchKernelGGL(add_block<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, (const T*)tmp, n1, R12,
+                           (int64_t)ldr, n1, n2);
+    }
     hipLaunchKernelGGL(zero_block<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, R, (int64_t)ldr, n1, n2, 0, n1);
     if (hipGetLastError() != hipSuccess) return TSQR_ERR_HIP;
-    return qr_rec<T>(m, n2, A2, lda, R22, ldr, ws, panel_bytes, part, st);
+    return qr_rec<T>(m, n2, A2, lda, R22, ldr, ws, panel_bytes, part, tmp, st, mixed);
 }
 
 template <typename T>
-int qr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t work_bytes, hipStream_t st) {
+int qr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t work_bytes, hipStream_t st,
+              bool mixed = false) {
     int e = check_args(m, n, A, lda, R, ldr);
     if (e) return e;
     const size_t need = qr_ws_bytes<T>(m, n);
     if (need == 0) return TSQR_ERR_SHAPE;
     if (work_bytes < need || !work) return TSQR_ERR_WORKSPACE;
-    if (n <= 32) return tsqr_core<T>(m, n, A, lda, R, ldr, work, st);
+    if (n <= 32) return tsqr_core<T>(m, n, A, lda, R, ldr, work, st, mixed);
     const size_t panel = align256(tsqr_ws_bytes<T>(m, 32));
+    const size_t parts = align256(tn_partial_elems(m, n) * sizeof(T));
     char* ws = reinterpret_cast<char*>(work);
-    return qr_rec<T>(m, n, A, lda, R, ldr, ws, panel, reinterpret_cast<T*>(ws + panel), st);
+    return qr_rec<T>(m, n, A, lda, R, ldr, ws, panel, reinterpret_cast<T*>(ws + panel),
+                     reinterpret_cast<T*>(ws + panel + parts), st, mixed);
 }
 
 template <typename T>
@@ -412,6 +442,10 @@ int qr_s_stream(int m, int n, float* A, int lda, float* R, int ldr, void* work, 
 int qr_d_stream(int m, int n, double* A, int lda, double* R, int ldr, void* work, size_t work_bytes, void* stream) {
     return qr_stream<double>(m, n, A, lda, R, ldr, work, work_bytes, (hipStream_t)stream);
 }
+int qr_mixed_s_stream(int m, int n, float* A, int lda, float* R, int ldr, void* work, size_t work_bytes,
+                      void* stream) {
+    return qr_stream<float>(m, n, A, lda, R, ldr, work, work_bytes, (hipStream_t)stream, true);
+}
 
 int tsqr_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int ldwork) {
     int e = check_args(m, n, A, lda, R, ldr);
@@ -456,6 +490,17 @@ int qr_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int ld
     e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
     if (e) return e;
     return qr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+}
+int qr_mixed_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int ldwork) {
+    int e = check_args(m, n, A, lda, R, ldr);
+    if (e) return e;
+    const size_t need = qr_ws_bytes<float>(m, n);
+    if (need == 0) return TSQR_ERR_SHAPE;
+    void* ws = nullptr;
+    size_t bytes = 0;
+    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
+    if (e) return e;
+    return qr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr, true);
 }
 int qr_d(int m, int n, double* A, int lda, double* R, int ldr, double* work, int ldwork) {
     int e = check_args(m, n, A, lda, R, ldr);
