@@ -40,6 +40,9 @@ EXPORTS = (
     "tsqr_mixed_stats",
     "qr_mixed_s",
     "qr_mixed_s_stream",
+    "tsqr_set_leaf_tiles",
+    "tsqr_leaf_tiles",
+    "tsqr_set_gemm_waves",
     "tsqr_batch_create",
     "tsqr_batch_workspace_bytes",
     "tsqr_batch_run",
@@ -110,11 +113,24 @@ def lib() -> ctypes.CDLL:
     L.tsqr_batch_stats.restype = i32
     L.tsqr_batch_destroy.argtypes = [vp]
     L.tsqr_batch_destroy.restype = i32
+    L.tsqr_set_leaf_tiles.argtypes = [i32, i32, i32, i32]
+    L.tsqr_set_leaf_tiles.restype = i32
+    L.tsqr_leaf_tiles.argtypes = [i32, i32, i32]
+    L.tsqr_leaf_tiles.restype = i32
+    L.tsqr_set_gemm_waves.argtypes = [i32]
+    L.tsqr_set_gemm_waves.restype = i32
     L.tsqr_timing_enable.argtypes = [i32]
     L.tsqr_timing_enable.restype = i32
     L.tsqr_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), i32]
     L.tsqr_timing_read.restype = i32
     _lib = L
+    # the autotuned superleaf table for this GPU, if one has been written
+    # (tsqr_amd/autotune.py); TSQR_AMD_NO_TUNING=1 keeps the built-in rule
+    if not os.environ.get("TSQR_AMD_NO_TUNING"):
+        from .autotune import DEFAULT_FILE, apply, load
+
+        if os.path.exists(DEFAULT_FILE):
+            apply(load(DEFAULT_FILE))
     return L
 
 

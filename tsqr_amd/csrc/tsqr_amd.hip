@@ -49,6 +49,26 @@ void timing_mark(int phase, bool begin, hipStream_t st) {
     }
 }
 
+// ------------------------------------------------------------------------
+// Tuning table (tsqr_set_leaf_tiles, tsqr_set_gemm_waves).
+// ------------------------------------------------------------------------
+namespace {
+std::atomic<int> g_leaf_tiles[2][3][32];  // zero-initialised: nothing tuned
+std::atomic<int> g_tn_waves{512};
+
+inline int width_class(int N) { return N <= 8 ? 0 : (N <= 16 ? 1 : 2); }
+inline int log2_floor(int64_t m) {
+    int k = 0;
+    while (k < 31 && (m >> (k + 1)) != 0) ++k;
+    return k;
+}
+}  // namespace
+
+int tuned_leaf_tiles(int dtype_index, int N, int m) {
+    if (dtype_index < 0 || dtype_index > 1 || m < 1) return 0;
+    return g_leaf_tiles[dtype_index][width_class(N)][log2_floor(m)].load(std::memory_order_relaxed);
+}
+
 namespace {
 
 struct Pool {
@@ -149,10 +169,12 @@ int tsqr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t w
 inline int split_cols(int n) { return 32 * ((n + 63) / 64); }
 
 // Rows per split-K chunk of R12 = Q1^T A2: enough chunks that the launch has
-// about 512 waves, each chunk a multiple of 16 rows and at least 64 rows.
+// about g_tn_waves waves (512 unless tuned), each chunk a multiple of 16
+// rows and at least 64 rows.
 inline int tn_chunk_rows(int m, int n1, int n2) {
     const int tiles = ((n1 + 63) / 64) * ((n2 + 63) / 64);
-    const int chunks = (512 + tiles - 1) / tiles;
+    const int waves = g_tn_waves.load(std::memory_order_relaxed);
+    const int chunks = (waves + tiles - 1) / tiles;
     int kc = (m + chunks - 1) / chunks;
     kc = (kc + 15) / 16 * 16;
     return kc < 64 ? 64 : kc;
@@ -410,6 +432,28 @@ int tsqr_mixed_stats(const void* work, int m, int n, int* flat, int* passes, int
         *check = ctl[2];
         *residual = ctl[1] ? e[1] : e[0];
     }
+    return TSQR_OK;
+}
+
+int tsqr_set_leaf_tiles(int dtype, int n, int log2m, int tiles) {
+    if ((dtype != TSQR_FLOAT && dtype != TSQR_DOUBLE) || n < 1 || n > 32 || log2m < 0 || log2m > 31 || tiles < 0 ||
+        tiles > 8)
+        return TSQR_ERR_ARG;
+    g_leaf_tiles[dtype == TSQR_FLOAT ? 0 : 1][width_class(padded_cols(n))][log2m].store(tiles);
+    return TSQR_OK;
+}
+
+int tsqr_leaf_tiles(int dtype, int m, int n) {
+    if (n < 1 || n > 32 || m < n) return TSQR_ERR_ARG;
+    const int N = padded_cols(n);
+    if (dtype == TSQR_FLOAT) return leaf_rows_for<float>(m, n) / (64 * rows_per_lane<float>(N));
+    if (dtype == TSQR_DOUBLE) return leaf_rows_for<double>(m, n) / (64 * rows_per_lane<double>(N));
+    return TSQR_ERR_ARG;
+}
+
+int tsqr_set_gemm_waves(int waves) {
+    if (waves < 1 || waves > 1 << 20) return TSQR_ERR_ARG;
+    g_tn_waves.store(waves);
     return TSQR_OK;
 }
 

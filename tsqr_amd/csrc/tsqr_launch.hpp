@@ -28,14 +28,22 @@ constexpr int super_tiles(int N) {
 // Waves the level-0 passes aim to keep busy: two per SIMD of a 256-CU part.
 constexpr int kTargetLeaves = 2048;
 
-// Superleaf height: about m / kTargetLeaves rows, a whole number of tiles,
-// between one tile and super_tiles tiles.  Large m gets full superleaves
-// (fewer, longer streams and a smaller tree); small m gets one-tile
-// superleaves so that the level-0 passes still fill the GPU.
+// Tiles per superleaf chosen by the autotuner (tsqr_set_leaf_tiles) for
+// (dtype index 0 float / 1 double, padded width N, floor(log2 m)); 0 when
+// the bucket has not been tuned.  Defined in tsqr_amd.hip.
+int tuned_leaf_tiles(int dtype_index, int N, int m);
+
+// Superleaf height: the tuned number of tiles for this shape if there is
+// one, else about m / kTargetLeaves rows, a whole number of tiles, between
+// one tile and super_tiles tiles.  Large m gets full superleaves (fewer,
+// longer streams and a smaller tree); small m gets one-tile superleaves so
+// that the level-0 passes still fill the GPU.
 template <typename T>
 inline int leaf_rows_for(int m, int n) {
     const int N = padded_cols(n);
     const int H = 64 * rows_per_lane<T>(N);
+    const int tuned = tuned_leaf_tiles(sizeof(T) == 4 ? 0 : 1, N, m);
+    if (tuned > 0) return (tuned < super_tiles<T>(N) ? tuned : super_tiles<T>(N)) * H;
     const int64_t want = ((int64_t)m + kTargetLeaves - 1) / kTargetLeaves;
     int64_t tiles = (want + H - 1) / H;
     if (tiles < 1) tiles = 1;
