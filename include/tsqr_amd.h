@@ -202,6 +202,21 @@ int tsqr_batch_destroy(tsqr_batch *plan);
  * affected. */
 int tsqr_set_leaf_path(int dtype, int path, double gate);
 
+/* Tuning table, replacing the reference's compile-time NUM_SM / BLOCK_SIZE
+ * (kernelQR.h:3-11); written by the autotuner (tsqr_amd/autotune.py), and
+ * callable directly.  Process-wide; a change affects calls planned after it,
+ * including the workspace size, so query tsqr_workspace_bytes afterwards.
+ *   tsqr_set_leaf_tiles  tiles per level-0 superleaf (1..8, clipped to what
+ *                        the tile shape allows; 0 = the built-in rule) for
+ *                        matrices of this dtype and padded width of n
+ *                        (8, 16 or 32) with floor(log2 m) == log2m;
+ *   tsqr_leaf_tiles      the tiles per superleaf a call would use now;
+ *   tsqr_set_gemm_waves  waves the split-K R12 = Q1^T A2 launch of the
+ *                        recursive QR aims for (default 512). */
+int tsqr_set_leaf_tiles(int dtype, int n, int log2m, int tiles);
+int tsqr_leaf_tiles(int dtype, int m, int n);
+int tsqr_set_gemm_waves(int waves);
+
 /* After a tsqr_*_stream call that used `work` has been enqueued: waits for
  * the device, counts the level-0 superleaves that took the Gram path and the
  * Householder path, and sets *flat to 1 if the call took the flat Gram path.

@@ -62,3 +62,28 @@ def test_workspace_sizing(lib):
     big = lib.tsqr_workspace_bytes(23887872, 32, 1)
     assert 0 < big < 23887872 * 32 * 4 // 6
     assert lib.qr_workspace_bytes(27648, 128, 2) >= lib.tsqr_workspace_bytes(27648, 32, 2)
+
+
+def test_compat_header_compiles_reference_style_caller(tmp_path):
+    """A caller written against the reference's templates (test_tsqr.cu:42-53
+    style) compiles unchanged against include/tsqr_compat.hpp."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("no g++")
+    src = tmp_path / "caller.cpp"
+    src.write_text(
+        '#include "tsqr_compat.hpp"\n'
+        "template <typename T> void run(int m, int n, T* A, T* R, T* work) {\n"
+        "    int ldwork = 2 * NUM_SM * BLOCK_SIZE;\n"
+        "    tsqr<T>(m, n, A, m, R, n, work, ldwork);\n"
+        "    qr<T>(m, 4 * n, A, m, R, 4 * n, work, ldwork);\n"
+        "}\n"
+        "template void run<float>(int, int, float*, float*, float*);\n"
+        "template void run<double>(int, int, double*, double*, double*);\n")
+    inc = os.path.join(ROOT, "include")
+    r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-I", inc, "-c", str(src), "-o",
+                        str(tmp_path / "caller.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

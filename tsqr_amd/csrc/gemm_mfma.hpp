@@ -156,17 +156,29 @@ __global__ __launch_bounds__(64) void gemm_tn_mfma(int m, int n1, int n2, const 
 }
 
 // C[i + j*ldc] = (accumulate ? C : 0) + sum over chunks of
-// part[k][i + j*n1], chunks in order.
+// part[k][i + j*n1].  256 threads per 16 outputs: thread (o, g) adds the
+// chunks k = g, g + 16, ... of output o in order, then the 16 group sums
+// are added in order of g, so the result is deterministic and no thread
+// walks more than chunks / 16 partials.
 template <typename T>
-__global__ void gemm_tn_reduce(int n1, int n2, int chunks, const T* __restrict__ part, T* __restrict__ C,
-                               int64_t ldc, int accumulate) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n1 * n2) return;
+__global__ __launch_bounds__(256) void gemm_tn_reduce(int n1, int n2, int chunks, const T* __restrict__ part,
+                                                      T* __restrict__ C, int64_t ldc, int accumulate) {
+    __shared__ T red[16][17];
+    const int o = threadIdx.x & 15, g = threadIdx.x >> 4;
+    const int idx = blockIdx.x * 16 + o;
+    const int total = n1 * n2;
     T s = T(0);
-    for (int k = 0; k < chunks; ++k) s += part[(int64_t)k * n1 * n2 + idx];
+    if (idx < total)
+        for (int k = g; k < chunks; k += 16) s += part[(int64_t)k * total + idx];
+    red[o][g] = s;
+    __syncthreads();
+    if (g != 0 || idx >= total) return;
+    T sum = T(0);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) sum += red[o][q];
     const int i = idx % n1, j = idx / n1;
     T* c = C + i + (int64_t)j * ldc;
-    *c = accumulate ? *c + s : s;
+    *c = accumulate ? *c + sum : sum;
 }
 
 // C[rows, j0:j0+64] -= X[rows, 0:K] B[0:K, j0:j0+64] for the 64-row block

@@ -219,7 +219,7 @@ int couple(int m, int n1, int n2, const T* A1, T* A2, int lda, T* C, int ldc, T*
     else
         hipLaunchKernelGGL((gemm_tn_mfma<T, false>), dim3(chunks, (n1 + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m,
                            n1, n2, A1, (int64_t)lda, (const T*)A2, (int64_t)lda, kc, part);
-    hipLaunchKernelGGL(gemm_tn_reduce<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, n1, n2, chunks,
+    hipLaunchKernelGGL(gemm_tn_reduce<T>, dim3((n1 * n2 + 15) / 16), dim3(256), 0, st, n1, n2, chunks,
                        (const T*)part, C, (int64_t)ldc, 0);
     if (timed) timing_mark(kPhaseGemmTN, false, st);
     if (timed) timing_mark(kPhaseGemmNN, true, st);
