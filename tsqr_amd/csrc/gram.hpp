@@ -188,10 +188,11 @@ __device__ __forceinline__ double sum_half(double v) {
 // Pass 1a of the Gram path, one wavefront per superleaf: G = A_b^T A_b in
 // FP64 (MFMA), written to side[leaf] (32 x 32, row-major).  A streaming
 // kernel: the only state is the 12 accumulator registers and the slabs in
-// flight.
+// flight.  Does nothing if skip[0] is set (the flat path succeeded).
 template <typename T, int NT, bool VEC>
 __global__ __launch_bounds__(64) void tsqr_gram_g(const TreeDesc d, const T* __restrict__ A, int lda,
-                                                   double* __restrict__ side) {
+                                                   double* __restrict__ side, const int* __restrict__ skip) {
+    if (skip[0]) return;  // the flat path already has R and R^{-1}
     const int lane = threadIdx.x;
     const int n = d.n;
     const int leaf = blockIdx.x;
@@ -367,7 +368,9 @@ template <typename T, bool VEC>
 __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restrict__ A, int lda,
                                                    const double* __restrict__ side, const T* __restrict__ blocks,
                                                    const int* __restrict__ flags, const int* __restrict__ ctl) {
-    const int leaf = blockIdx.x;
+    // Highest superleaves first: the streaming R pass reads them last, so
+    // part of them is still in the memory-side cache.
+    const int leaf = (int)gridDim.x - 1 - (int)blockIdx.x;
     if (flags[leaf]) return;
     const int lane = threadIdx.x;
     const int n = d.n;
@@ -577,6 +580,222 @@ __global__ __launch_bounds__(256) void tsqr_gram_flat(const TreeDesc d, int npar
     __syncthreads();
     if (t >= 64) return;
     flat_factor<T>(L, d.n, R, ldr, side + (int64_t)d.lv[0].tiles * 1024, ctl, gate, rkeep, t);
+}
+
+// ---------------------------------------------------------------------------
+// Flat Gram path, streaming pass (tsqr_gram_stream).
+//
+// The whole-matrix Gram G = A^T A does not need per-superleaf sums, so the
+// rows are streamed by a persistent grid instead: workgroup b (4 waves) of P
+// owns wave indices w = 4b + 0..3, and wave w accumulates the chunks (64
+// rows of FP32, 32 of FP64) w, w + 4P, w + 8P, ... in FP64 MFMA registers (the interleaving keeps
+// every wave on nearby addresses and all of them busy to the end).  The
+// workgroup then sums its 4 waves in LDS (in wave order) into partial[b].
+// Workgroups form groups of kStreamGroup; the last of a group to finish
+// (ticket; no wave ever waits on another) sums the group's partials in index
+// order into partial[P + g].  tsqr_gram_flat then sums the ceil(P /
+// kStreamGroup) group partials in order and factors the result.  Every sum
+// runs in a fixed order, so the result does not depend on which workgroup
+// arrives last.  P depends only on m (stream_groups, plan.hpp).
+
+// Slabs of 16 rows per streamed chunk: 64 rows of FP32 or 32 rows of FP64,
+// so that a chunk and its prefetched successor take 64 VGPRs either way.
+template <typename T>
+constexpr int stream_slabs() {
+    return sizeof(T) == 4 ? 4 : 2;
+}
+
+// Rows [r0, r0 + 4) of column c, all inside the matrix (no row check); zero
+// for the padding columns c >= ncols.
+template <typename T, bool VEC>
+__device__ __forceinline__ Raw4<T> load_full4(const T* __restrict__ base, int64_t ld, int r0, int c, int ncols) {
+    Raw4<T> o;
+    if (c < ncols) {
+        const T* p = base + (int64_t)c * ld + r0;
+        if constexpr (VEC && sizeof(T) == 4) {
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            const f4 q = *reinterpret_cast<const f4*>(p);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) o.v[t] = q[t];
+        } else if constexpr (VEC) {
+            typedef double d2 __attribute__((ext_vector_type(2)));
+            const d2 q0 = *reinterpret_cast<const d2*>(p);
+            const d2 q1 = *reinterpret_cast<const d2*>(p + 2);
+            o.v[0] = q0[0];
+            o.v[1] = q0[1];
+            o.v[2] = q1[0];
+            o.v[3] = q1[1];
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) o.v[t] = p[t];
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) o.v[t] = T(0);
+    }
+    return o;
+}
+
+// FP64 MFMA Gram update of one chunk held as S slabs of 16 rows (lane l:
+// rows 16q + 4(l/16) .. +3 of columns l%16 and 16 + l%16).
+template <typename T, int NT, int S>
+__device__ __forceinline__ void gram_chunk(const Raw4<T> (&c0)[S], const Raw4<T> (&c1)[S], v4d& a00, v4d& a01,
+                                           v4d& a11) {
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const double x0 = (double)c0[q].v[t];
+            a00 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, x0, a00, 0, 0, 0);
+            if constexpr (NT == 2) {
+                const double x1 = (double)c1[q].v[t];
+                a01 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, x1, a01, 0, 0, 0);
+                a11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, x1, a11, 0, 0, 0);
+            }
+        }
+    }
+}
+
+// Sums `count` 32 x 32 FP64 blocks at src in index order (256 threads):
+// out[e] is entry t + 256 e of the sum.  Four blocks are in flight per step.
+__device__ __forceinline__ void sum_blocks(const double* __restrict__ src, int count, int t, double (&out)[4]) {
+    double s[4] = {0, 0, 0, 0};
+    int b = 0;
+#pragma unroll 1
+    for (; b + 4 <= count; b += 4) {
+        double v[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[q][e] = src[(int64_t)(b + q) * 1024 + t + 256 * e];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) s[e] += v[q][e];
+    }
+    for (; b < count; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s[e] += src[(int64_t)b * 1024 + t + 256 * e];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) out[e] = s[e];
+}
+
+// One ticket per workgroup: returns (in every thread) whether this
+// workgroup is the last of `count` to arrive at *tk.  The caller's global
+// writes are released before the ticket and, for the last arriver, the
+// other workgroups' writes are acquired after it.
+__device__ __forceinline__ bool last_group(int* tk, int count, int* bcast) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const int old = __hip_atomic_fetch_add(tk, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *bcast = old;
+    }
+    __syncthreads();
+    const bool last = *bcast == count - 1;
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return last;
+}
+
+// partial: P + ceil(P / kStreamGroup) blocks of 32 x 32 FP64; tk: the
+// kCtlStreamTickets ticket words, zero on entry.
+template <typename T, int NT, bool VEC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void tsqr_gram_stream(const TreeDesc d, const T* __restrict__ A, int lda,
+                                                         double* __restrict__ partial, int* __restrict__ tk) {
+    __shared__ GramLds L;
+    __shared__ int bcast;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wv = t >> 6;
+    const int m = d.m, n = d.n;
+    const int P = gridDim.x;
+    const int W = 4 * P;
+    const int w = 4 * blockIdx.x + wv;
+    const int col = lane & 15;
+    const int rq = 4 * (lane >> 4);
+    constexpr int S = stream_slabs<T>();
+    constexpr int CR = 16 * S;  // rows per chunk
+    const int nfull = m / CR;   // complete chunks
+    v4d a00 = {0, 0, 0, 0}, a01 = {0, 0, 0, 0}, a11 = {0, 0, 0, 0};
+    if (w < nfull) {
+        Raw4<T> c0[S], c1[S];
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            c0[q] = load_full4<T, VEC>(A, lda, CR * w + 16 * q + rq, col, n);
+            if constexpr (NT == 2) c1[q] = load_full4<T, VEC>(A, lda, CR * w + 16 * q + rq, col + 16, n);
+        }
+#pragma unroll 1
+        for (int ch = w;;) {
+            const int next = ch + W;
+            const bool more = next < nfull;
+            Raw4<T> p0[S], p1[S];
+            if (more) {
+#pragma unroll
+                for (int q = 0; q < S; ++q) {
+                    p0[q] = load_full4<T, VEC>(A, lda, CR * next + 16 * q + rq, col, n);
+                    if constexpr (NT == 2) p1[q] = load_full4<T, VEC>(A, lda, CR * next + 16 * q + rq, col + 16, n);
+                }
+            }
+            gram_chunk<T, NT, S>(c0, c1, a00, a01, a11);
+            if (!more) break;
+            ch = next;
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+                c0[q] = p0[q];
+                if constexpr (NT == 2) c1[q] = p1[q];
+            }
+        }
+    }
+    if (m % CR && w == nfull % W) {  // the short last chunk, bounds-checked loads
+        Raw4<T> c0[S], c1[S];
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            c0[q] = load_raw4<T, false>(A, lda, CR * nfull + 16 * q + rq, col, m, n);
+            if constexpr (NT == 2) c1[q] = load_raw4<T, false>(A, lda, CR * nfull + 16 * q + rq, col + 16, m, n);
+        }
+        gram_chunk<T, NT, S>(c0, c1, a00, a01, a11);
+    }
+    // workgroup sum in wave order; lane l, register v holds row 4v + l/16,
+    // column l%16 of each 16 x 16 block
+    const int i0 = lane >> 4;
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+        if (wv == s) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 4 * v + i0;
+                const double g01 = NT == 2 ? a01[v] : 0.0;
+                const double g11 = NT == 2 ? a11[v] : 0.0;
+                if (s == 0) {
+                    L.G[i][col] = a00[v];
+                    L.G[i][16 + col] = g01;
+                    L.G[16 + col][i] = g01;
+                    L.G[16 + i][16 + col] = g11;
+                } else {
+                    L.G[i][col] += a00[v];
+                    L.G[i][16 + col] += g01;
+                    L.G[16 + col][i] += g01;
+                    L.G[16 + i][16 + col] += g11;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* mine = partial + (int64_t)blockIdx.x * 1024;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int idx = t + 256 * e;
+        mine[idx] = L.G[idx >> 5][idx & 31];
+    }
+    const int g = blockIdx.x / kStreamGroup;
+    const int g0 = g * kStreamGroup;
+    const int members = P - g0 < kStreamGroup ? P - g0 : kStreamGroup;
+    if (!last_group(tk + g, members, &bcast)) return;
+    double s[4];
+    sum_blocks(partial + (int64_t)g0 * 1024, members, t, s);
+    double* gsum = partial + (int64_t)(P + g) * 1024;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) gsum[t + 256 * e] = s[e];
 }
 
 }  // namespace tsqr

@@ -93,15 +93,42 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d) {
 //   flags    one int per level-0 superleaf: 0 Gram path, 1 Householder path
 //   list     count, then the indices of the Householder-path superleaves
 //   ctl      control words: ctl[0] = 1 when the whole matrix took the flat
-//            Gram path; ctl[1], ctl[2] for the mixed path (mixed.hpp)
-//   partial  min(superleaves, kGramPartials) blocks of 32 x 32 FP64 partial
-//            sums of the superleaf Grams, plus kGramPartials2 for a second
-//            stage when there are more than kGramPartials2 of them
+//            Gram path; ctl[1], ctl[2] for the mixed path (mixed.hpp);
+//            ctl[64 .. 64 + kCtlStreamTickets) the tickets of the streaming
+//            Gram kernel
+//   partial  32 x 32 FP64 partial sums: min(superleaves, kGramPartials)
+//            blocks of superleaf Gram sums plus kGramPartials2 for a second
+//            stage when there are more than kGramPartials2 of them, or the
+//            streaming Gram kernel's P + ceil(P / 32) blocks (P =
+//            stream_partials(m)), whichever is more
 //   side     32 x 32 FP64 per superleaf (the Gram matrix, then R^{-1}), then
 //            three more: the whole matrix's R^{-1} on the flat path (T0),
 //            its R (T0+1) and the mixed path's residuals (T0+2)
 constexpr int kGramPartials = 128;
 constexpr int kGramPartials2 = 8;
+constexpr int kCtlWords = 128;         // control words, including the stream tickets
+constexpr int kCtlStreamTickets = 64;  // ctl[64 ..]: tickets of the streaming Gram
+
+// The streaming Gram kernel (tsqr_gram_stream, gram.hpp) runs P workgroups
+// of 4 waves, P = stream_groups(m): at most kStreamMaxGroups, and at least
+// four 64-row chunks per wave so that the loads of one chunk overlap the
+// MFMAs of the previous one.  Its partial sums take P blocks plus one per
+// group of kStreamGroup workgroups.
+constexpr int kStreamMaxGroups = 1024;
+constexpr int kStreamGroup = 32;
+static_assert(kStreamMaxGroups / kStreamGroup + 1 <= kCtlStreamTickets, "stream tickets do not fit");
+
+inline int stream_groups(int64_t m) {
+    const int64_t chunks = (m + 63) / 64;
+    int64_t p = (chunks + 15) / 16;
+    if (p < 1) p = 1;
+    return (int)(p < kStreamMaxGroups ? p : kStreamMaxGroups);
+}
+
+inline size_t stream_partials(int64_t m) {
+    const int p = stream_groups(m);
+    return (size_t)(p + (p + kStreamGroup - 1) / kStreamGroup);
+}
 
 struct WsLayout {
     size_t blocks, tickets, flags, list, ctl, partial, side, total;
@@ -115,10 +142,12 @@ inline WsLayout ws_layout(const TreeDesc& d, size_t elem) {
     w.flags = w.tickets + align256((size_t)d.total_ctr * sizeof(int));
     w.list = w.flags + align256(leaves * sizeof(int));
     w.ctl = w.list + align256((leaves + 1) * sizeof(int));
-    w.partial = w.ctl + 256;
+    w.partial = w.ctl + kCtlWords * sizeof(int);
     const size_t parts = leaves < (size_t)kGramPartials ? leaves : (size_t)kGramPartials;
     const size_t parts2 = parts > (size_t)kGramPartials2 ? (size_t)kGramPartials2 : 0;
-    w.side = w.partial + (parts + parts2) * 1024 * sizeof(double);
+    const size_t sparts = stream_partials(d.m);
+    const size_t nparts = parts + parts2 > sparts ? parts + parts2 : sparts;
+    w.side = w.partial + nparts * 1024 * sizeof(double);
     w.total = w.side + align256((leaves + 3) * 1024 * sizeof(double));
     return w;
 }

@@ -2,21 +2,22 @@
 //
 // Launch sequence for one call (m > H):
 //   init          tickets = 0, superleaf flags = path, control words = 0
-//   gram_g        (auto path) FP64 Gram matrix of every superleaf
-//   gram_partial, gram_flat
-//                 (auto path) sum of all superleaf Grams and its Cholesky;
-//                 if the whole matrix passes the gate this is the flat Gram
-//                 path (Q = A R^{-1}) and the per-superleaf kernels and the
-//                 tree below return at once
-//   gram_chol     (auto path) Cholesky R, R^{-1} and the gate per
-//                 superleaf; flags the superleaves it rejects
+//   gram_stream   (auto path) FP64 Gram matrix of the whole matrix by a
+//                 persistent grid, its Cholesky and the gate in the same
+//                 launch; if the whole matrix passes the gate this is the
+//                 flat Gram path (Q = A R^{-1}) and the per-superleaf kernels
+//                 and the tree below return at once
+//   gram_g        (auto path when the flat gate rejects, gram-tree path) FP64
+//                 Gram matrix of every superleaf
+//   gram_chol     Cholesky R, R^{-1} and the gate per superleaf; flags the
+//                 superleaves it rejects
 //   super_r       Householder R of the flagged superleaves
 //   tree_up       levels >= 1, one launch, ticketed walk to the root
 //   down          levels root-1 .. 1, one launch each
 //   gram_q        M = R^{-1} S, then Q = A M, for the Gram superleaves
 //   super_q       Householder Q for the flagged superleaves
 // The CholeskyQR2 path replaces the Gram kernels by mixed_gram (FP32,
-// BF16x3) or gram_g (FP64), the flat reduction and Cholesky, mixed_q +
+// BF16x3) with the flat reduction and Cholesky, or gram_stream (FP64), then mixed_q +
 // mixed_check (pass 1) and mixed_q + mixed_check (pass 2, a no-op unless
 // the check asks for it); every superleaf is flagged Householder so that the
 // tree path takes over if the flat gate rejects.
@@ -44,7 +45,7 @@ __global__ void init_flags(int* __restrict__ tickets, int nt, int* __restrict__ 
         flags[i] = value;
         if (value) list[1 + i] = i;
     }
-    if (i < 64) ctl[i] = 0;
+    if (i < kCtlWords) ctl[i] = 0;
     if (i == 0) list[0] = value ? nf : 0;
 }
 
@@ -93,31 +94,41 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     constexpr int NT = N > 16 ? 2 : 1;
 
     const int ninit = d.total_ctr > leaves ? d.total_ctr : leaves;
-    hipLaunchKernelGGL(init_flags, dim3((ninit + 63 + 255) / 256), dim3(256), 0, st, tickets, d.total_ctr, flags,
-                       list, leaves, force, ctl);
+    hipLaunchKernelGGL(init_flags, dim3((ninit + kCtlWords - 1 + 255) / 256), dim3(256), 0, st, tickets,
+                       d.total_ctr, flags, list, leaves, force, ctl);
     // Householder superleaf kernels: a grid-stride loop over the list
     const int super_grid = leaves < 1024 ? leaves : 1024;
     double* rkeep = side + (size_t)(leaves + 1) * 1024;
     if (timed) timing_mark(kPhaseLeafR, true, st);
+    // streaming Gram: sgroups workgroups, then sgroup_parts group partials
+    int* stream_tk = ctl + kCtlStreamTickets;
+    const int sgroups = stream_groups(d.m);
+    const int sgroup_parts = (sgroups + kStreamGroup - 1) / kStreamGroup;
+    double* stream_sum = partial + (size_t)sgroups * 1024;
     if (mixed) {
-        if constexpr (sizeof(T) == 4)
+        if constexpr (sizeof(T) == 4) {
             hipLaunchKernelGGL((tsqr_mixed_gram<NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const float*)A, lda,
                                side);
-        else
-            hipLaunchKernelGGL((tsqr_gram_g<T, NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const T*)A, lda, side);
-        int np = 0;
-        const double* parts = reduce_grams(leaves, side, partial, nullptr, &np, st);
-        hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, np, parts, R, ldr, side, ctl, o.gate,
-                           rkeep);
-    }
-    if (gram) {
-        hipLaunchKernelGGL((tsqr_gram_g<T, NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const T*)A, lda, side);
-        if (leaves >= 2 && o.path == kLeafAuto) {
             int np = 0;
             const double* parts = reduce_grams(leaves, side, partial, nullptr, &np, st);
             hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, np, parts, R, ldr, side, ctl,
-                               o.gate, (double*)nullptr);
+                               o.gate, rkeep);
+        } else {
+            hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
+                               partial, stream_tk);
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, sgroup_parts,
+                               (const double*)stream_sum, R, ldr, side, ctl, o.gate, rkeep);
         }
+    }
+    if (gram) {
+        if (leaves >= 2 && o.path == kLeafAuto) {
+            hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
+                               partial, stream_tk);
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, sgroup_parts,
+                               (const double*)stream_sum, R, ldr, side, ctl, o.gate, (double*)nullptr);
+        }
+        hipLaunchKernelGGL((tsqr_gram_g<T, NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const T*)A, lda, side,
+                           (const int*)ctl);
         hipLaunchKernelGGL((tsqr_gram_chol<T>), dim3((leaves + 1) / 2), dim3(64), 0, st, d, R, ldr, blocks, side,
                            flags, list, (const int*)ctl, o.gate);
     }
