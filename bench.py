@@ -434,6 +434,23 @@ def main():
         }
     phase_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in phases.items() if v[1]}
 
+    # Errors of one untimed factorisation of the same input, from the
+    # library's device-side FP64 checker (tsqr_check_*: n x n Gram reductions
+    # of A, Q and A - QR over all m rows).
+    errors = None
+    if is_tsqr and n <= 32:
+        from tsqr_amd._lib import check
+
+        g.manual_seed(0)
+        A.uniform_(-1.0, 1.0, generator=g)
+        A0 = A.clone()
+        step()
+        torch.cuda.synchronize()
+        errors = check(m, n, code, A0.data_ptr(), m, A.data_ptr(), m, R.data_ptr(), n, stream.cuda_stream)
+        errors = {k: float(f"{v:.3e}") for k, v in errors.items()}
+        errors["source"] = "tsqr_check (device FP64)"
+        del A0
+
     out = {
         "metric": "TSQR HBM GB/s (% of peak); batched QR/s mixed vs FP32 at equal error",
         "value": round(value, 2),
@@ -458,6 +475,7 @@ def main():
             "device_copy_GBs": copy_gbs,
             "superleaves": dict(zip(("gram", "householder", "flat"), stats)) if kind == "tsqr" else None,
             "mixed": mixed,
+            "errors": errors,
         },
         "roofline": roofline,
     }

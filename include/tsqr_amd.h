@@ -224,6 +224,37 @@ int tsqr_set_gemm_waves(int waves);
 int tsqr_leaf_stats(const void *work, int m, int n, int dtype, int *gram_leaves, int *householder_leaves,
                     int *flat);
 
+/* R only (no counterpart in the reference, whose tsqr always overwrites A
+ * with Q; TallShinnyQR.h:86-149 is the call it otherwise mirrors).  Writes
+ * R (n x n, ldr, zeros below the diagonal, the same R as tsqr_s / tsqr_d up
+ * to rounding) and never writes A: the Q pass and the down sweep are
+ * skipped, so A is read once on the flat Gram path (twice for FP64, whose
+ * second CholeskyQR pass streams the Gram matrix of A R1^-1 without
+ * storing it).  Workspace as tsqr_s / tsqr_d (tsqr_workspace_bytes). */
+int tsqr_r_s(int m, int n, const float *A, int lda, float *R, int ldr, float *work, int ldwork);
+int tsqr_r_d(int m, int n, const double *A, int lda, double *R, int ldr, double *work, int ldwork);
+int tsqr_r_s_stream(int m, int n, const float *A, int lda, float *R, int ldr, void *work, size_t work_bytes,
+                    void *stream);
+int tsqr_r_d_stream(int m, int n, const double *A, int lda, double *R, int ldr, void *work, size_t work_bytes,
+                    void *stream);
+
+/* Device-side FP64 checker of a factorisation, replaces check_QR_accuracy
+ * (utils.h:317-455), whose m x m gesvd workspace limits it to m <= ~46k.
+ * A is the input (m x n, lda), Q and R the computed factors (ldq, ldr),
+ * n <= 32.  Three streaming FP64 reductions on the GPU form the n x n Gram
+ * matrices A^T A, Q^T Q and E^T E with E = A - QR (formed in FP64 on the
+ * matrix cores); the host then takes their traces and symmetric
+ * eigenvalues.  Writes
+ *   norms[0] = ||A - QR||_F / ||A||_F   norms[1] = ||I - Q^T Q||_F
+ *   norms[2] = ||A - QR||_2 / ||A||_2   norms[3] = ||I - Q^T Q||_2
+ * (the reference prints norms[2] and norms[3] / n, utils.h:380-381).
+ * Runs on `stream` (0 = default), uses an internal per-device workspace
+ * and waits for its own work before returning. */
+int tsqr_check_s(int m, int n, const float *A, int lda, const float *Q, int ldq, const float *R, int ldr,
+                 double *norms, void *stream);
+int tsqr_check_d(int m, int n, const double *A, int lda, const double *Q, int ldq, const double *R, int ldr,
+                 double *norms, void *stream);
+
 /* Per-phase timing (no counterpart in the reference, which only times whole
  * calls with CUDA events, test_tsqr.cu:62-90).  While enabled, every call
  * brackets each launch phase with HIP events on its stream:

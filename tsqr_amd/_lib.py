@@ -50,6 +50,12 @@ EXPORTS = (
     "tsqr_batch_set_mixed",
     "tsqr_batch_stats",
     "tsqr_batch_destroy",
+    "tsqr_check_s",
+    "tsqr_check_d",
+    "tsqr_r_s",
+    "tsqr_r_d",
+    "tsqr_r_s_stream",
+    "tsqr_r_d_stream",
 )
 
 
@@ -79,12 +85,12 @@ def lib() -> ctypes.CDLL:
     for f in (L.tsqr_workspace_bytes, L.qr_workspace_bytes):
         f.argtypes = [i32, i32, i32]
         f.restype = sz
-    for name in ("tsqr_s", "tsqr_d", "qr_s", "qr_d", "tsqr_mixed_s", "qr_mixed_s"):
+    for name in ("tsqr_s", "tsqr_d", "qr_s", "qr_d", "tsqr_mixed_s", "qr_mixed_s", "tsqr_r_s", "tsqr_r_d"):
         f = getattr(L, name)
         f.argtypes = [i32, i32, vp, i32, vp, i32, vp, i32]
         f.restype = i32
     for name in ("tsqr_s_stream", "tsqr_d_stream", "qr_s_stream", "qr_d_stream", "tsqr_mixed_s_stream",
-                 "qr_mixed_s_stream"):
+                 "qr_mixed_s_stream", "tsqr_r_s_stream", "tsqr_r_d_stream"):
         f = getattr(L, name)
         f.argtypes = [i32, i32, vp, i32, vp, i32, vp, sz, vp]
         f.restype = i32
@@ -119,6 +125,10 @@ def lib() -> ctypes.CDLL:
     L.tsqr_leaf_tiles.restype = i32
     L.tsqr_set_gemm_waves.argtypes = [i32]
     L.tsqr_set_gemm_waves.restype = i32
+    for name in ("tsqr_check_s", "tsqr_check_d"):
+        f = getattr(L, name)
+        f.argtypes = [i32, i32, vp, i32, vp, i32, vp, i32, ctypes.POINTER(ctypes.c_double), vp]
+        f.restype = i32
     L.tsqr_timing_enable.argtypes = [i32]
     L.tsqr_timing_enable.restype = i32
     L.tsqr_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), i32]
@@ -194,3 +204,19 @@ def tsqr_workspace_bytes(m: int, n: int, dtype_code: int) -> int:
 
 def qr_workspace_bytes(m: int, n: int, dtype_code: int) -> int:
     return int(lib().qr_workspace_bytes(m, n, dtype_code))
+
+
+CHECK_KEYS = ("backward_fro", "orth_fro", "backward_2", "orth_2")
+
+
+def check(m: int, n: int, dtype_code: int, A_ptr: int, lda: int, Q_ptr: int, ldq: int, R_ptr: int, ldr: int,
+          stream_ptr: int = 0) -> dict:
+    """Device-side FP64 check of a factorisation (tsqr_check_s/d): the
+    relative backward error ||A - QR|| / ||A|| and the orthogonality
+    ||I - Q^T Q||, Frobenius and spectral, all pointers on the device."""
+    norms = (ctypes.c_double * 4)()
+    fn = lib().tsqr_check_s if dtype_code == TSQR_FLOAT else lib().tsqr_check_d
+    st = fn(m, n, A_ptr, lda, Q_ptr, ldq, R_ptr, ldr, norms, stream_ptr or None)
+    if st != TSQR_OK:
+        raise TsqrError(st, "tsqr_check")
+    return dict(zip(CHECK_KEYS, list(norms)))

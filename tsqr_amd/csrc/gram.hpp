@@ -188,14 +188,13 @@ __device__ __forceinline__ double sum_half(double v) {
 // Pass 1a of the Gram path, one wavefront per superleaf: G = A_b^T A_b in
 // FP64 (MFMA), written to side[leaf] (32 x 32, row-major).  A streaming
 // kernel: the only state is the 12 accumulator registers and the slabs in
-// flight.  Does nothing if skip[0] is set (the flat path succeeded).
+// flight.  A grid-stride loop over the superleaves; does nothing if skip[0]
+// is set (the flat path succeeded).
 template <typename T, int NT, bool VEC>
-__global__ __launch_bounds__(64) void tsqr_gram_g(const TreeDesc d, const T* __restrict__ A, int lda,
-                                                   double* __restrict__ side, const int* __restrict__ skip) {
-    if (skip[0]) return;  // the flat path already has R and R^{-1}
+__device__ __forceinline__ void gram_g_leaf(const TreeDesc& d, const T* __restrict__ A, int lda,
+                                            double* __restrict__ side, int leaf) {
     const int lane = threadIdx.x;
     const int n = d.n;
-    const int leaf = blockIdx.x;
     const LevelDesc& L0 = d.lv[0];
     const int rows = leaf_rows_of(L0, leaf);
     const T* base = A + (int64_t)leaf * L0.tile_rows;
@@ -275,7 +274,16 @@ __global__ __launch_bounds__(64) void tsqr_gram_g(const TreeDesc d, const T* __r
     }
 }
 
-// Pass 1b, two superleaves per wavefront (one per half-wave):
+template <typename T, int NT, bool VEC>
+__global__ __launch_bounds__(64) void tsqr_gram_g(const TreeDesc d, const T* __restrict__ A, int lda,
+                                                   double* __restrict__ side, const int* __restrict__ skip) {
+    if (skip[0]) return;  // the flat path already has R and R^{-1}
+#pragma unroll 1
+    for (int leaf = blockIdx.x; leaf < d.lv[0].tiles; leaf += gridDim.x) gram_g_leaf<T, NT, VEC>(d, A, lda, side, leaf);
+}
+
+// Pass 1b, two superleaves per wavefront (one per half-wave), a grid-stride
+// loop over the superleaf pairs:
 // R = chol(G), X = R^{-1}, kappa_est = ||R||_F ||R^{-1}||_F.
 // Success: R rounded to T goes to the superleaf's slot in its parent block
 // (or to the caller's R if the superleaf is the root) and X (FP64, 32 x 32,
@@ -285,18 +293,15 @@ __global__ __launch_bounds__(64) void tsqr_gram_g(const TreeDesc d, const T* __r
 // order), and nothing else is written; the Householder superleaf kernels
 // take over.
 template <typename T>
-__global__ __launch_bounds__(64) void tsqr_gram_chol(const TreeDesc d, T* __restrict__ R, int ldr,
-                                                      T* __restrict__ blocks, double* __restrict__ side,
-                                                      int* __restrict__ flags, int* __restrict__ list,
-                                                      const int* __restrict__ ctl, double gate) {
-    __shared__ GramLds Lh[2];
-    if (ctl[0]) return;  // the flat path already has R and R^{-1}
+__device__ __forceinline__ void gram_chol_pair(const TreeDesc& d, T* __restrict__ R, int ldr, T* __restrict__ blocks,
+                                               double* __restrict__ side, int* __restrict__ flags,
+                                               int* __restrict__ list, double gate, GramLds (&Lh)[2], int pair) {
     const int lane = threadIdx.x;
     const int half = lane >> 5;
     const int c = lane & 31;
     const int n = d.n;
     const LevelDesc& L0 = d.lv[0];
-    const int leaf = 2 * blockIdx.x + half;
+    const int leaf = 2 * pair + half;
     const bool real = leaf < L0.tiles;
     GramLds& L = Lh[half];
     double* G = side + (int64_t)(real ? leaf : 0) * 1024;
@@ -351,6 +356,21 @@ __global__ __launch_bounds__(64) void tsqr_gram_chol(const TreeDesc d, T* __rest
     // X row-major: lane c holds column c
 #pragma unroll
     for (int r = 0; r < 32; ++r) G[r * 32 + c] = x[r];
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void tsqr_gram_chol(const TreeDesc d, T* __restrict__ R, int ldr,
+                                                      T* __restrict__ blocks, double* __restrict__ side,
+                                                      int* __restrict__ flags, int* __restrict__ list,
+                                                      const int* __restrict__ ctl, double gate) {
+    __shared__ GramLds Lh[2];
+    if (ctl[0]) return;  // the flat path already has R and R^{-1}
+    const int pairs = (d.lv[0].tiles + 1) / 2;
+#pragma unroll 1
+    for (int pair = blockIdx.x; pair < pairs; pair += gridDim.x) {
+        gram_chol_pair<T>(d, R, ldr, blocks, side, flags, list, gate, Lh, pair);
+        wave_sync();  // Lh is reused by the next pair
+    }
 }
 
 // Pass 2 of the Gram path, one wavefront per superleaf.
@@ -495,26 +515,34 @@ __global__ __launch_bounds__(256) void tsqr_gram_partial(int leaves, const doubl
     for (int q = 0; q < 4; ++q) partial[(int64_t)blockIdx.x * 1024 + threadIdx.x + 256 * q] = acc[q];
 }
 
-// L.G = sum of the nparts 32 x 32 FP64 partials, added in order (256
-// threads, thread t owns entries t + 256 e; four chains per entry so that
-// the loads overlap).  The caller synchronises before reading L.G.
+// L.G = sum of the nparts 32 x 32 FP64 partials, added in index order
+// (kSumThreads = 256 threads, thread t owns entries t + 256 e).  The
+// partials were written by other CUs, so every load goes to memory: eight
+// partials (32 loads per thread) are in flight at a time.  The caller
+// synchronises before reading L.G.
+constexpr int kSumThreads = 256;
 __device__ __forceinline__ void sum_partials(GramLds& L, int nparts, const double* __restrict__ partial, int t) {
-    double s[4][4] = {};
+    double s[4] = {0, 0, 0, 0};
     int p = 0;
 #pragma unroll 1
-    for (; p + 4 <= nparts; p += 4) {
+    for (; p + 8 <= nparts; p += 8) {
+        double v[8][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 8; ++q)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) s[e][q] += partial[(int64_t)(p + q) * 1024 + t + 256 * e];
+            for (int e = 0; e < 4; ++e) v[q][e] = partial[(int64_t)(p + q) * 1024 + t + 256 * e];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) s[e] += v[q][e];
     }
     for (; p < nparts; ++p)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) s[e][0] += partial[(int64_t)p * 1024 + t + 256 * e];
+        for (int e = 0; e < 4; ++e) s[e] += partial[(int64_t)p * 1024 + t + 256 * e];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int idx = t + 256 * e;
-        L.G[idx >> 5][idx & 31] = (s[e][0] + s[e][1]) + (s[e][2] + s[e][3]);
+        L.G[idx >> 5][idx & 31] = s[e];
     }
 }
 
@@ -565,12 +593,12 @@ __device__ __forceinline__ bool flat_factor(GramLds& L, int n, T* __restrict__ R
     return ok;
 }
 
-// Flat Gram path, stage 2 (one workgroup of 256 threads): G = sum of the
+// Flat Gram path, stage 2 (one workgroup of kSumThreads threads): G = sum of the
 // partials in order, then flat_factor with X = side block T0.  If the gate
 // passes, the per-superleaf kernels and the tree do nothing and the Q pass
 // forms Q = A R^{-1}; otherwise ctl[0] stays 0 and the tree path runs.
 template <typename T>
-__global__ __launch_bounds__(256) void tsqr_gram_flat(const TreeDesc d, int nparts,
+__global__ __launch_bounds__(kSumThreads) void tsqr_gram_flat(const TreeDesc d, int nparts,
                                                         const double* __restrict__ partial, T* __restrict__ R,
                                                         int ldr, double* __restrict__ side, int* __restrict__ ctl,
                                                         double gate, double* __restrict__ rkeep) {
@@ -657,19 +685,20 @@ __device__ __forceinline__ void gram_chunk(const Raw4<T> (&c0)[S], const Raw4<T>
 }
 
 // Sums `count` 32 x 32 FP64 blocks at src in index order (256 threads):
-// out[e] is entry t + 256 e of the sum.  Four blocks are in flight per step.
+// out[e] is entry t + 256 e of the sum.  Eight blocks are in flight per
+// step (the blocks come from other CUs, so every load goes to memory).
 __device__ __forceinline__ void sum_blocks(const double* __restrict__ src, int count, int t, double (&out)[4]) {
     double s[4] = {0, 0, 0, 0};
     int b = 0;
 #pragma unroll 1
-    for (; b + 4 <= count; b += 4) {
-        double v[4][4];
+    for (; b + 8 <= count; b += 8) {
+        double v[8][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 8; ++q)
 #pragma unroll
             for (int e = 0; e < 4; ++e) v[q][e] = src[(int64_t)(b + q) * 1024 + t + 256 * e];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 8; ++q)
 #pragma unroll
             for (int e = 0; e < 4; ++e) s[e] += v[q][e];
     }
@@ -695,6 +724,62 @@ __device__ __forceinline__ bool last_group(int* tk, int count, int* bcast) {
     const bool last = *bcast == count - 1;
     if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return last;
+}
+
+// End of a streaming Gram launch (256 threads, 4 waves, P = gridDim.x
+// workgroups): each wave holds its FP64 MFMA accumulators of the blocks
+// (0,0), (0,1), (1,1) of a 32 x 32 Gram matrix (lane l, register v: row
+// 4v + l/16, column l%16 of the block).  The workgroup sums its waves in
+// wave order into partial[blockIdx.x]; the last workgroup of each group of
+// kStreamGroup (ticket tk[g]) sums the group's partials in index order into
+// partial[P + g].
+template <int NT>
+__device__ __forceinline__ void stream_reduce(GramLds& L, const v4d& a00, const v4d& a01, const v4d& a11,
+                                              double* __restrict__ partial, int* __restrict__ tk, int* bcast) {
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wv = t >> 6;
+    const int col = lane & 15;
+    const int i0 = lane >> 4;
+    const int P = gridDim.x;
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+        if (wv == s) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 4 * v + i0;
+                const double g01 = NT == 2 ? a01[v] : 0.0;
+                const double g11 = NT == 2 ? a11[v] : 0.0;
+                if (s == 0) {
+                    L.G[i][col] = a00[v];
+                    L.G[i][16 + col] = g01;
+                    L.G[16 + col][i] = g01;
+                    L.G[16 + i][16 + col] = g11;
+                } else {
+                    L.G[i][col] += a00[v];
+                    L.G[i][16 + col] += g01;
+                    L.G[16 + col][i] += g01;
+                    L.G[16 + i][16 + col] += g11;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* mine = partial + (int64_t)blockIdx.x * 1024;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int idx = t + 256 * e;
+        mine[idx] = L.G[idx >> 5][idx & 31];
+    }
+    const int g = blockIdx.x / kStreamGroup;
+    const int g0 = g * kStreamGroup;
+    const int members = P - g0 < kStreamGroup ? P - g0 : kStreamGroup;
+    if (!last_group(tk + g, members, bcast)) return;
+    double sum[4];
+    sum_blocks(partial + (int64_t)g0 * 1024, members, t, sum);
+    double* gsum = partial + (int64_t)(P + g) * 1024;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) gsum[t + 256 * e] = sum[e];
 }
 
 // partial: P + ceil(P / kStreamGroup) blocks of 32 x 32 FP64; tk: the
@@ -755,47 +840,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
         }
         gram_chunk<T, NT, S>(c0, c1, a00, a01, a11);
     }
-    // workgroup sum in wave order; lane l, register v holds row 4v + l/16,
-    // column l%16 of each 16 x 16 block
-    const int i0 = lane >> 4;
-#pragma unroll 1
-    for (int s = 0; s < 4; ++s) {
-        if (wv == s) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = 4 * v + i0;
-                const double g01 = NT == 2 ? a01[v] : 0.0;
-                const double g11 = NT == 2 ? a11[v] : 0.0;
-                if (s == 0) {
-                    L.G[i][col] = a00[v];
-                    L.G[i][16 + col] = g01;
-                    L.G[16 + col][i] = g01;
-                    L.G[16 + i][16 + col] = g11;
-                } else {
-                    L.G[i][col] += a00[v];
-                    L.G[i][16 + col] += g01;
-                    L.G[16 + col][i] += g01;
-                    L.G[16 + i][16 + col] += g11;
-                }
-            }
-        }
-        __syncthreads();
-    }
-    double* mine = partial + (int64_t)blockIdx.x * 1024;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int idx = t + 256 * e;
-        mine[idx] = L.G[idx >> 5][idx & 31];
-    }
-    const int g = blockIdx.x / kStreamGroup;
-    const int g0 = g * kStreamGroup;
-    const int members = P - g0 < kStreamGroup ? P - g0 : kStreamGroup;
-    if (!last_group(tk + g, members, &bcast)) return;
-    double s[4];
-    sum_blocks(partial + (int64_t)g0 * 1024, members, t, s);
-    double* gsum = partial + (int64_t)(P + g) * 1024;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) gsum[t + 256 * e] = s[e];
+    stream_reduce<NT>(L, a00, a01, a11, partial, tk, &bcast);
 }
 
 }  // namespace tsqr

@@ -400,11 +400,11 @@ __device__ __forceinline__ void mixed_check_body(GramLds& L, int n, T* __restric
     }
 }
 
-// Check kernel (one workgroup of 256 threads): G = sum of the nparts
+// Check kernel (one workgroup of kSumThreads threads): G = sum of the nparts
 // partials (in order), then mixed_check_body with X = side block T0, R1 in
 // side block T0+1 and the residuals in side block T0+2.
 template <typename T>
-__global__ __launch_bounds__(256) void tsqr_mixed_check(const TreeDesc d, int nparts,
+__global__ __launch_bounds__(kSumThreads) void tsqr_mixed_check(const TreeDesc d, int nparts,
                                                         const double* __restrict__ partial, T* __restrict__ R,
                                                         int ldr, double* __restrict__ side, int* __restrict__ ctl,
                                                         int pass, double tol) {

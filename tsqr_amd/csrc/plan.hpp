@@ -94,8 +94,8 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d) {
 //   list     count, then the indices of the Householder-path superleaves
 //   ctl      control words: ctl[0] = 1 when the whole matrix took the flat
 //            Gram path; ctl[1], ctl[2] for the mixed path (mixed.hpp);
-//            ctl[64 .. 64 + kCtlStreamTickets) the tickets of the streaming
-//            Gram kernel
+//            ctl[64 .. 128) and ctl[128 .. 192) the tickets of two streaming
+//            Gram passes
 //   partial  32 x 32 FP64 partial sums: min(superleaves, kGramPartials)
 //            blocks of superleaf Gram sums plus kGramPartials2 for a second
 //            stage when there are more than kGramPartials2 of them, or the
@@ -106,8 +106,9 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d) {
 //            its R (T0+1) and the mixed path's residuals (T0+2)
 constexpr int kGramPartials = 128;
 constexpr int kGramPartials2 = 8;
-constexpr int kCtlWords = 128;         // control words, including the stream tickets
-constexpr int kCtlStreamTickets = 64;  // ctl[64 ..]: tickets of the streaming Gram
+constexpr int kCtlWords = 192;          // control words, including the stream tickets
+constexpr int kCtlStreamTickets = 64;   // ctl[64 ..]: tickets of the streaming Gram
+constexpr int kCtlStreamTickets2 = 128; // ctl[128 ..]: tickets of a second streaming pass
 
 // The streaming Gram kernel (tsqr_gram_stream, gram.hpp) runs P workgroups
 // of 4 waves, P = stream_groups(m): at most kStreamMaxGroups, and at least
@@ -128,6 +129,12 @@ inline int stream_groups(int64_t m) {
 inline size_t stream_partials(int64_t m) {
     const int p = stream_groups(m);
     return (size_t)(p + (p + kStreamGroup - 1) / kStreamGroup);
+}
+
+// Workspace of the device-side checker (check.hpp): three partial areas of
+// stream_partials(m) blocks and three sets of stream tickets.
+inline size_t check_workspace_bytes(int64_t m) {
+    return 3 * stream_partials(m) * 1024 * sizeof(double) + 3 * kCtlStreamTickets * sizeof(int);
 }
 
 struct WsLayout {

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -76,14 +77,15 @@ struct Pool {
     size_t bytes = 0;
 };
 std::mutex g_pool_mu;
-Pool g_pool[64];
+Pool g_pool[64];        // the reference-shaped entry points
+Pool g_check_pool[64];  // the device-side checker
 
-// Per-device internal workspace for the reference-shaped entry points.
-int pool_get(size_t bytes, void** out) {
+// Per-device internal workspace (grows, never shrinks).
+int pool_get(size_t bytes, void** out, Pool* pools = g_pool) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return TSQR_ERR_HIP;
     std::lock_guard<std::mutex> lock(g_pool_mu);
-    Pool& p = g_pool[dev];
+    Pool& p = pools[dev];
     if (p.bytes < bytes) {
         if (p.ptr) {
             if (hipDeviceSynchronize() != hipSuccess) return TSQR_ERR_HIP;
@@ -142,23 +144,31 @@ LaunchOpts mixed_opts() {
 
 // Runs one TSQR; work must hold tsqr_ws_bytes<T>(m, n) bytes.
 template <typename T>
-int tsqr_core(int m, int n, T* A, int lda, T* R, int ldr, void* work, hipStream_t st, bool mixed = false) {
+int tsqr_core(int m, int n, T* A, int lda, T* R, int ldr, void* work, hipStream_t st, bool mixed = false,
+              bool r_only = false) {
     TreeDesc d;
     if (!plan_tsqr<T>(m, n, &d)) return TSQR_ERR_SHAPE;
-    return tsqr_launch<T>(d, A, lda, R, ldr, work, mixed ? mixed_opts() : current_opts<T>(), st);
+    LaunchOpts o = mixed ? mixed_opts() : current_opts<T>();
+    o.r_only = r_only ? 1 : 0;
+    return tsqr_launch<T>(d, A, lda, R, ldr, work, o, st);
 }
 
 template <typename T>
 int tsqr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t work_bytes, hipStream_t st,
-                bool mixed = false) {
+                bool mixed = false, bool r_only = false) {
     int e = check_args(m, n, A, lda, R, ldr);
     if (e) return e;
     if (n > 32) return TSQR_ERR_SHAPE;
     const size_t need = tsqr_ws_bytes<T>(m, n);
     if (need == 0) return TSQR_ERR_SHAPE;
     if (work_bytes < need || (need > 0 && !work)) return TSQR_ERR_WORKSPACE;
-    return tsqr_core<T>(m, n, A, lda, R, ldr, work, st, mixed);
+    return tsqr_core<T>(m, n, A, lda, R, ldr, work, st, mixed, r_only);
 }
+
+// R only (tsqr_r_*): the reference-shaped entry with the library's pool when
+// the caller's ldwork is short.  A is read and never written.
+template <typename T>
+int tsqr_r_ref(int m, int n, const T* A, int lda, T* R, int ldr, T* work, int ldwork);
 
 // ------------------------------------------------------------------------
 // Recursive blocked QR.
@@ -280,6 +290,90 @@ int qr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t wor
                      reinterpret_cast<T*>(ws + panel + parts), st, mixed);
 }
 
+// Eigenvalues of the symmetric n x n matrix S (row-major, ld 32, n <= 32;
+// destroyed) by cyclic Jacobi rotations, into ev[0..n).
+void sym_eigenvalues(double (*S)[32], int n, double* ev) {
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < n; ++i) {
+            diag += S[i][i] * S[i][i];
+            for (int j = i + 1; j < n; ++j) off += S[i][j] * S[i][j];
+        }
+        if (off <= 1e-32 * diag || off == 0.0) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                if (S[p][q] == 0.0) continue;
+                // rotation that zeroes S[p][q] (Golub and Van Loan, Algorithm 8.5.1)
+                const double tau = (S[q][q] - S[p][p]) / (2.0 * S[p][q]);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double skp = S[k][p], skq = S[k][q];
+                    S[k][p] = c * skp - sn * skq;
+                    S[k][q] = sn * skp + c * skq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double spk = S[p][k], sqk = S[q][k];
+                    S[p][k] = c * spk - sn * sqk;
+                    S[q][k] = sn * spk + c * sqk;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i) ev[i] = S[i][i];
+}
+
+// norms[0..3] of tsqr_check_* from the device Gram reductions.
+template <typename T>
+int check_impl(int m, int n, const T* A, int lda, const T* Q, int ldq, const T* R, int ldr, double* norms,
+               hipStream_t st) {
+    if (!A || !Q || !R || !norms || n < 1 || m < n || lda < m || ldq < m || ldr < n) return TSQR_ERR_ARG;
+    if (n > 32) return TSQR_ERR_SHAPE;
+    void* work = nullptr;
+    int e = pool_get(check_workspace_bytes(m), &work, g_check_pool);
+    if (e) return e;
+    size_t off[3];
+    e = check_launch<T>(m, n, A, lda, Q, ldq, R, ldr, work, off, st);
+    if (e) return e;
+    const int groups = (stream_groups(m) + kStreamGroup - 1) / kStreamGroup;
+    std::vector<double> host((size_t)3 * groups * 1024);
+    for (int k = 0; k < 3; ++k)
+        if (hipMemcpyAsync(host.data() + (size_t)k * groups * 1024, reinterpret_cast<const char*>(work) + off[k],
+                           (size_t)groups * 1024 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
+            return TSQR_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return TSQR_ERR_HIP;
+    double G[3][32][32];
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < 32; ++i)
+            for (int j = 0; j < 32; ++j) {
+                double sum = 0.0;
+                for (int g = 0; g < groups; ++g) sum += host[((size_t)k * groups + g) * 1024 + i * 32 + j];
+                G[k][i][j] = sum;
+            }
+    double ev[32];
+    double trA = 0.0, trE = 0.0, fQ = 0.0;
+    for (int i = 0; i < n; ++i) {
+        trA += G[0][i][i];
+        trE += G[2][i][i];
+        for (int j = 0; j < n; ++j) {
+            const double d = G[1][i][j] - (i == j ? 1.0 : 0.0);
+            fQ += d * d;
+        }
+    }
+    auto lmax = [&](int k, bool shift) {
+        for (int i = 0; i < n; ++i) G[k][i][i] -= shift ? 1.0 : 0.0;
+        sym_eigenvalues(G[k], n, ev);
+        double mx = 0.0;
+        for (int i = 0; i < n; ++i) mx = std::fmax(mx, shift ? std::fabs(ev[i]) : ev[i]);
+        return mx;
+    };
+    const double a2 = lmax(0, false), q2 = lmax(1, true), e2 = lmax(2, false);
+    norms[0] = trA > 0.0 ? std::sqrt(std::fmax(trE, 0.0) / trA) : 0.0;
+    norms[1] = std::sqrt(fQ);
+    norms[2] = a2 > 0.0 ? std::sqrt(e2 / a2) : 0.0;
+    norms[3] = q2;
+    return TSQR_OK;
+}
+
 template <typename T>
 int with_ref_workspace(int m, int n, T* work, int ldwork, size_t need, void** ws, size_t* bytes) {
     const size_t have = (work && ldwork > 0) ? (size_t)ldwork * (size_t)n * sizeof(T) : 0;
@@ -290,6 +384,19 @@ int with_ref_workspace(int m, int n, T* work, int ldwork, size_t need, void** ws
     }
     *bytes = need;
     return pool_get(need, ws);
+}
+
+template <typename T>
+int tsqr_r_ref(int m, int n, const T* A, int lda, T* R, int ldr, T* work, int ldwork) {
+    int e = check_args(m, n, A, lda, R, ldr);
+    if (e) return e;
+    const size_t need = tsqr_ws_bytes<T>(m, n);
+    if (need == 0) return TSQR_ERR_SHAPE;
+    void* ws = nullptr;
+    size_t bytes = 0;
+    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
+    if (e) return e;
+    return tsqr_stream<T>(m, n, const_cast<T*>(A), lda, R, ldr, ws, bytes, nullptr, false, true);
 }
 
 }  // namespace
@@ -316,6 +423,16 @@ const char* tsqr_status_string(int status) {
         default:
             return "unknown status";
     }
+}
+
+int tsqr_check_s(int m, int n, const float* A, int lda, const float* Q, int ldq, const float* R, int ldr,
+                 double* norms, void* stream) {
+    return check_impl<float>(m, n, A, lda, Q, ldq, R, ldr, norms, (hipStream_t)stream);
+}
+
+int tsqr_check_d(int m, int n, const double* A, int lda, const double* Q, int ldq, const double* R, int ldr,
+                 double* norms, void* stream) {
+    return check_impl<double>(m, n, A, lda, Q, ldq, R, ldr, norms, (hipStream_t)stream);
 }
 
 int tsqr_timing_enable(int enable) {
@@ -491,6 +608,22 @@ int qr_mixed_s_stream(int m, int n, float* A, int lda, float* R, int ldr, void* 
     return qr_stream<float>(m, n, A, lda, R, ldr, work, work_bytes, (hipStream_t)stream, true);
 }
 
+int tsqr_r_s(int m, int n, const float* A, int lda, float* R, int ldr, float* work, int ldwork) {
+    return tsqr_r_ref<float>(m, n, A, lda, R, ldr, work, ldwork);
+}
+int tsqr_r_d(int m, int n, const double* A, int lda, double* R, int ldr, double* work, int ldwork) {
+    return tsqr_r_ref<double>(m, n, A, lda, R, ldr, work, ldwork);
+}
+int tsqr_r_s_stream(int m, int n, const float* A, int lda, float* R, int ldr, void* work, size_t work_bytes,
+                    void* stream) {
+    return tsqr_stream<float>(m, n, const_cast<float*>(A), lda, R, ldr, work, work_bytes, (hipStream_t)stream,
+                              false, true);
+}
+int tsqr_r_d_stream(int m, int n, const double* A, int lda, double* R, int ldr, void* work, size_t work_bytes,
+                    void* stream) {
+    return tsqr_stream<double>(m, n, const_cast<double*>(A), lda, R, ldr, work, work_bytes, (hipStream_t)stream,
+                               false, true);
+}
 int tsqr_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int ldwork) {
     int e = check_args(m, n, A, lda, R, ldr);
     if (e) return e;

@@ -218,10 +218,11 @@ __global__ __launch_bounds__(64) void tsqr_tree_up(const TreeDesc d, T* __restri
     tree_up_body<T, N, RPL>(d, R, ldr, blocks, tickets, L, blockIdx.x);
 }
 
-// A single tile (m <= H): factor, R to the caller, Q over A.
+// A single tile (m <= H): factor, R to the caller, Q over A (unless form_q
+// is 0).
 template <typename T, int N, int RPL, bool VEC>
 __global__ __launch_bounds__(64) void tsqr_single(const TreeDesc d, T* __restrict__ A, int lda, T* __restrict__ R,
-                                                   int ldr) {
+                                                   int ldr, int form_q) {
     __shared__ TileLds<T, N> L;
     const int lane = threadIdx.x;
     const int n = d.n;
@@ -230,6 +231,7 @@ __global__ __launch_bounds__(64) void tsqr_single(const TreeDesc d, T* __restric
     load_rows<T, N, RPL, VEC>(x, A, lda, rows, n, lane);
     factor_tile<T, N, RPL>(x, L, lane);
     store_r<T, N>(L, R, ldr, n, n, lane);
+    if (!form_q) return;  // R only: A stays as it was
     form_q_tile<T, N, RPL>(x, L, lane);
     store_rows<T, N, RPL, VEC>(x, A, lda, rows, n, lane);
 }

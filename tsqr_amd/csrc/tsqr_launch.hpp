@@ -69,9 +69,10 @@ enum LeafPath {
 };
 
 struct LaunchOpts {
-    int path;     // LeafPath
-    double gate;  // largest accepted ||R||_F ||R^{-1}||_F on the Gram path
-    double tol;   // mixed path: largest accepted ||Q^T Q - I||_F
+    int path;        // LeafPath
+    double gate;     // largest accepted ||R||_F ||R^{-1}||_F on the Gram path
+    double tol;      // mixed path: largest accepted ||Q^T Q - I||_F
+    int r_only = 0;  // R only: A is read, never written (tsqr_r_*)
 };
 
 // Optional per-phase timing (tsqr_timing_* in the C ABI).  When enabled,
@@ -87,6 +88,24 @@ enum Phase {
 };
 bool timing_enabled();
 void timing_mark(int phase, bool begin, hipStream_t st);
+
+// 16-byte vector loads are possible at every column start.
+inline bool aligned16(const void* p, int ld, size_t elem) {
+    return ((uintptr_t)p % 16 == 0) && (((size_t)ld * elem) % 16 == 0);
+}
+
+// Enqueues the device-side checker's three Gram reductions (check.hpp) on
+// st; work holds check_workspace_bytes(m) bytes.  On completion the group
+// sums of A^T A, Q^T Q and E^T E (E = A - QR) start at byte offsets[0..2]
+// of work: ceil(stream_groups(m) / kStreamGroup) blocks of 32 x 32 FP64
+// each, to be added in order.  n <= 32.
+template <typename T>
+int check_launch(int m, int n, const T* A, int lda, const T* Q, int ldq, const T* R, int ldr, void* work,
+                 size_t* offsets, hipStream_t st);
+extern template int check_launch<float>(int, int, const float*, int, const float*, int, const float*, int, void*,
+                                        size_t*, hipStream_t);
+extern template int check_launch<double>(int, int, const double*, int, const double*, int, const double*, int,
+                                         void*, size_t*, hipStream_t);
 
 // Enqueues one TSQR (plan d) on stream st; work holds
 // tree_workspace_bytes(d, sizeof(T)) bytes.  Returns a TSQR_* status.

@@ -22,17 +22,23 @@
 // the check asks for it); every superleaf is flagged Householder so that the
 // tree path takes over if the flat gate rejects.
 // A single tile (m <= H) is one tsqr_single launch.
+// R only (LaunchOpts::r_only, the tsqr_r_* entry points): the sequence stops
+// after the tree walk up, so A is only read.  The FP64 CholeskyQR2 path then
+// forms the second pass's Gram matrix of A R1^{-1} by streaming A
+// (tsqr_resid_stream without a residual) instead of writing Q1.
 #pragma once
 #include "../../include/tsqr_amd.h"
+#include "check.hpp"
 #include "mixed.hpp"
 #include "super.hpp"
 #include "tsqr_launch.hpp"
 
 namespace tsqr {
 
-inline bool aligned16(const void* p, int ld, size_t elem) {
-    return ((uintptr_t)p % 16 == 0) && (((size_t)ld * elem) % 16 == 0);
-}
+// Workgroups of the grid-stride fallback kernels (gram_g, gram_chol): enough
+// to fill the GPU when they run, few enough that a launch which finds the
+// flat path taken costs little.
+constexpr int kFallbackGrid = 4096;
 
 namespace {
 // tickets = 0; flags = value; the Householder list is empty (value 0) or
@@ -74,7 +80,8 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     const bool timed = timing_enabled();
     if (d.lv[0].rows <= d.H) {
         if (timed) timing_mark(kPhaseLeafR, true, st);
-        hipLaunchKernelGGL((tsqr_single<T, N, RPL, VEC>), dim3(1), dim3(64), 0, st, d, A, lda, R, ldr);
+        hipLaunchKernelGGL((tsqr_single<T, N, RPL, VEC>), dim3(1), dim3(64), 0, st, d, A, lda, R, ldr,
+                           o.r_only ? 0 : 1);
         if (timed) timing_mark(kPhaseLeafR, false, st);
         return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
     }
@@ -88,8 +95,11 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     double* partial = reinterpret_cast<double*>(base + w.partial);
     double* side = reinterpret_cast<double*>(base + w.side);
     const int leaves = d.lv[0].tiles;
-    const bool mixed = o.path == kLeafCholQR2;
-    const bool gram = o.path == kLeafAuto || o.path == kLeafGramTree;
+    // R only, FP32: the flat FP64 Gram (exact products) needs no second
+    // pass, so the CholeskyQR2 setting runs as AUTO
+    const int path = (o.r_only && sizeof(T) == 4 && o.path == kLeafCholQR2) ? kLeafAuto : o.path;
+    const bool mixed = path == kLeafCholQR2;
+    const bool gram = path == kLeafAuto || path == kLeafGramTree;
     const int force = gram ? 0 : 1;
     constexpr int NT = N > 16 ? 2 : 1;
 
@@ -105,32 +115,46 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     const int sgroups = stream_groups(d.m);
     const int sgroup_parts = (sgroups + kStreamGroup - 1) / kStreamGroup;
     double* stream_sum = partial + (size_t)sgroups * 1024;
+    const double* X0 = side + (size_t)leaves * 1024;  // R^{-1} of the flat path
     if (mixed) {
         if constexpr (sizeof(T) == 4) {
             hipLaunchKernelGGL((tsqr_mixed_gram<NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const float*)A, lda,
                                side);
             int np = 0;
             const double* parts = reduce_grams(leaves, side, partial, nullptr, &np, st);
-            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, np, parts, R, ldr, side, ctl,
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, np, parts, R, ldr, side, ctl,
                                o.gate, rkeep);
         } else {
             hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
                                partial, stream_tk);
-            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, sgroup_parts,
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
                                (const double*)stream_sum, R, ldr, side, ctl, o.gate, rkeep);
+            if (o.r_only) {
+                // second CholeskyQR pass without writing Q1 = A R1^{-1}: the
+                // Gram matrix of the product streamed from A, then the check
+                // (R = R1, or R2 R1 with R2 = chol(Q1^T Q1))
+                hipLaunchKernelGGL((tsqr_resid_stream<T, double, NT, VEC, false>), dim3(sgroups), dim3(256), 0, st,
+                                   d.m, d.n, (const T*)nullptr, 0, (const T*)A, lda, (const double*)X0, 32, 1,
+                                   partial, ctl + kCtlStreamTickets2);
+                hipLaunchKernelGGL(tsqr_mixed_check<T>, dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
+                                   (const double*)stream_sum, R, ldr, side, ctl, 1, o.tol);
+            }
         }
     }
     if (gram) {
-        if (leaves >= 2 && o.path == kLeafAuto) {
+        if (leaves >= 2 && path == kLeafAuto) {
             hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
                                partial, stream_tk);
-            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(256), 0, st, d, sgroup_parts,
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
                                (const double*)stream_sum, R, ldr, side, ctl, o.gate, (double*)nullptr);
         }
-        hipLaunchKernelGGL((tsqr_gram_g<T, NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const T*)A, lda, side,
+        // grid-stride kernels: on the flat path they only read ctl[0]
+        const int gram_grid = leaves < kFallbackGrid ? leaves : kFallbackGrid;
+        const int chol_grid = (leaves + 1) / 2 < kFallbackGrid ? (leaves + 1) / 2 : kFallbackGrid;
+        hipLaunchKernelGGL((tsqr_gram_g<T, NT, VEC>), dim3(gram_grid), dim3(64), 0, st, d, (const T*)A, lda, side,
                            (const int*)ctl);
-        hipLaunchKernelGGL((tsqr_gram_chol<T>), dim3((leaves + 1) / 2), dim3(64), 0, st, d, R, ldr, blocks, side,
-                           flags, list, (const int*)ctl, o.gate);
+        hipLaunchKernelGGL((tsqr_gram_chol<T>), dim3(chol_grid), dim3(64), 0, st, d, R, ldr, blocks, side, flags, list,
+                           (const int*)ctl, o.gate);
     }
     hipLaunchKernelGGL((tsqr_super_r<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st, d,
                        (const T*)A, lda, R, ldr, blocks, (const int*)list, (const int*)ctl);
@@ -141,6 +165,7 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
                            tickets, (const int*)ctl);
         if (timed) timing_mark(kPhaseTree, false, st);
     }
+    if (o.r_only) return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
     if (timed) timing_mark(kPhaseDown, true, st);
     for (int l = d.levels - 2; l >= 1; --l)
         hipLaunchKernelGGL((tsqr_down<T, N, RPL>), dim3(d.lv[l].tiles), dim3(64), 0, st, d, l, blocks,
@@ -153,7 +178,7 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
                                (const int*)ctl, pass);
             int np = 0;
             const double* parts = reduce_grams(leaves, side, partial, ctl + (pass == 1 ? 0 : 1), &np, st);
-            hipLaunchKernelGGL(tsqr_mixed_check<T>, dim3(1), dim3(256), 0, st, d, np, parts, R, ldr, side, ctl, pass,
+            hipLaunchKernelGGL(tsqr_mixed_check<T>, dim3(1), dim3(kSumThreads), 0, st, d, np, parts, R, ldr, side, ctl, pass,
                                o.tol);
         }
     }
