@@ -70,3 +70,23 @@ def test_qr_mixed_reference_entry():
     st, Q2, R2, _ = call("qr_mixed", A, ref_shaped=True, ldwork=2 * 108 * 256)
     assert st == 0
     assert np.array_equal(Q1, Q2) and np.array_equal(R1, R2)
+
+
+def test_qr_mixed_r_and_coupling_against_fp64():
+    """R against FP64 LAPACK after sign normalisation, and the first
+    coupling block R12 against the FP64 product of the returned Q1 and the
+    input A2 (the BF16x3 coupling products keep FP32 accuracy)."""
+    A = O.gen_normal(50000, 128, np.float32, seed=21)
+    st, Q, R, _ = call("qr_mixed", A)
+    assert st == 0
+    Qo, Ro = O.lapack_qr(A)
+    kappa = np.linalg.cond(A.astype(np.float64))
+    mm = O.metrics(A, Q, R, Ro, Qo)
+    bound = 8 * 128 * U32
+    print(f"qr_mixed 50000x128: r_err={mm['r_err']:.3e} kappa={kappa:.3f}", flush=True)
+    assert mm["r_err"] <= bound * kappa * kappa
+    Q1 = Q[:, :64].astype(np.float64)
+    A2 = A[:, 64:].astype(np.float64)
+    err = np.linalg.norm(R[:64, 64:] - Q1.T @ A2) / np.linalg.norm(A2)
+    print(f"qr_mixed R12 vs Q1^T A2: {err:.3e}", flush=True)
+    assert err <= 8 * 64 * U32

@@ -1,6 +1,8 @@
 // gemm_mfma.hpp -- the two coupling products of one recursive QR level on the
 // matrix cores (v_mfma_f32_16x16x4f32 for float, v_mfma_f64_16x16x4f64 for
-// double; products and sums in the working precision, like the reference).
+// double; products and sums in the working precision, like the reference;
+// BF16x3 split products on v_mfma_f32_16x16x32_bf16 for the mixed-precision
+// entry points).
 //
 // The recursive QR (block_QR.h:72-95) couples its column halves by one pass
 // of block classical Gram-Schmidt:
@@ -25,6 +27,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "bf16x3.hpp"
 #include "wave.hpp"
 
 namespace tsqr {
@@ -233,6 +236,139 @@ __global__ __launch_bounds__(64) void gemm_nn_mfma(int m, int K, int N, const T*
 #pragma unroll
             for (int t = 0; t < 4; ++t) c.v[t] = acc[t][jt][v];
             store_four<T, VEC>(c, C, ldc, r0 + 4 * M::row(v, lq), j0 + 16 * jt + li, m, N);
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision coupling (FP32 data, the qr_mixed entry points): the same
+// two products on the BF16 matrix cores with BF16x3 split operands
+// (bf16x3.hpp), FP32 accumulation, 32 rows (tn) or 32 inner columns (nn)
+// per v_mfma_f32_16x16x32_bf16.  Grids and outputs are those of
+// gemm_tn_mfma / gemm_nn_mfma.
+
+// Rows [r, r+8) of column c of X, zero outside rows < m and columns < ncols.
+template <bool VEC>
+__device__ __forceinline__ void load_eight(float (&x)[8], const float* __restrict__ X, int64_t ld, int64_t r, int c,
+                                           int64_t m, int ncols) {
+    const Four<float> a = load_four<float, VEC>(X, ld, r, c, m, ncols);
+    const Four<float> b = load_four<float, VEC>(X, ld, r + 4, c, m, ncols);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        x[t] = a.v[t];
+        x[4 + t] = b.v[t];
+    }
+}
+
+// part[chunk] for the 64 x 64 tile (blockIdx.y, blockIdx.z) of Q^T A: lane
+// (li, lq) feeds rows 8 lq .. 8 lq + 7 of each 32-row step, column li of the
+// tile's 16-column groups, as both operands.
+template <bool VEC>
+__global__ __launch_bounds__(64) void gemm_tn_bf16x3(int m, int n1, int n2, const float* __restrict__ Q, int64_t ldq,
+                                                     const float* __restrict__ A, int64_t lda, int kc,
+                                                     float* __restrict__ part) {
+    const int lane = threadIdx.x;
+    const int li = lane & 15, lq = lane >> 4;
+    const int i0 = blockIdx.y * 64, j0 = blockIdx.z * 64;
+    const int64_t kbeg = (int64_t)blockIdx.x * kc;
+    const int64_t kend = kbeg + kc < m ? kbeg + kc : m;
+    f4v acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = f4v{0, 0, 0, 0};
+#pragma unroll 1
+    for (int64_t r0 = kbeg; r0 < kend; r0 += 32) {
+        const int64_t r = r0 + 8 * lq;
+        Split3 sq[4], sx[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            float v[8];
+            load_eight<VEC>(v, Q, ldq, r, i0 + 16 * a + li, kend, n1);
+            sq[a] = split3(v);
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            float v[8];
+            load_eight<VEC>(v, A, lda, r, j0 + 16 * b + li, kend, n2);
+            sx[b] = split3(v);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = mma_bf16x3(sq[a], sx[b], acc[a][b]);
+    }
+    float* out = part + (int64_t)blockIdx.x * n1 * n2;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + 16 * a + 4 * lq + v, j = j0 + 16 * b + li;
+                if (i < n1 && j < n2) out[i + (int64_t)j * n1] = acc[a][b][v];
+            }
+}
+
+// C[rows, j0:j0+64] -= X B for the 64-row block blockIdx.x, as gemm_nn_mfma.
+// MFMA t (t = 0..3) takes rows r0 + 4 i + t (i = 0..15) of X as its A
+// operand: lane (li, lq) loads rows r0 + 4 li .. +3 of the eight columns
+// k0 + 8 lq + s, s = 0..7, so every load is 16 contiguous bytes.  Its output
+// register v is row r0 + 16 lq + 4 v + t, column j0 + 16 jt + li.
+template <bool VEC>
+__global__ __launch_bounds__(64) void gemm_nn_bf16x3(int m, int K, int N, const float* __restrict__ X, int64_t ldx,
+                                                     const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
+                                                     int64_t ldc) {
+    const int lane = threadIdx.x;
+    const int li = lane & 15, lq = lane >> 4;
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    const int j0 = blockIdx.y * 64;
+    f4v acc[4][4];  // [t][jt]
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const Four<float> c = load_four<float, VEC>(C, ldc, r0 + 16 * lq + 4 * v, j0 + 16 * jt + li, m, N);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t][jt][v] = c.v[t];
+        }
+#pragma unroll 1
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        Split3 sx[4], sb[4];
+        {
+            float xv[4][8];  // [t][s]: X[r0 + 4 li + t][k0 + 8 lq + s]
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const Four<float> f = load_four<float, VEC>(X, ldx, r0 + 4 * li, k0 + 8 * lq + s, m, K);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) xv[t][s] = f.v[t];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) sx[t] = split3(xv[t]);
+        }
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) {
+            const int j = j0 + 16 * jt + li;
+            float bv[8];  // -B[k0 + 8 lq + s][j]
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int k = k0 + 8 * lq + s;
+                bv[s] = (k < K && j < N) ? -B[k + (int64_t)j * ldb] : 0.f;
+            }
+            sb[jt] = split3(bv);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int jt = 0; jt < 4; ++jt) acc[t][jt] = mma_bf16x3(sx[t], sb[jt], acc[t][jt]);
+    }
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            Four<float> c;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) c.v[t] = acc[t][jt][v];
+            store_four<float, VEC>(c, C, ldc, r0 + 16 * lq + 4 * v, j0 + 16 * jt + li, m, N);
         }
 }
 

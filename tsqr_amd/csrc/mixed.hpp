@@ -35,44 +35,10 @@
 // pass, 2 second Cholesky broke down).  The FP64 side block T0+2 holds
 // e1, e2.
 #pragma once
+#include "bf16x3.hpp"
 #include "gram.hpp"
 
 namespace tsqr {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f4v __attribute__((ext_vector_type(4)));
-
-struct Split3 {
-    bf16x8 h, m, l;
-};
-
-__device__ __forceinline__ Split3 split3(const float (&x)[8]) {
-    Split3 s;
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        const __bf16 h = (__bf16)x[t];
-        const float r = x[t] - (float)h;
-        const __bf16 m = (__bf16)r;
-        s.h[t] = h;
-        s.m[t] = m;
-        s.l[t] = (__bf16)(r - (float)m);
-    }
-    return s;
-}
-
-__device__ __forceinline__ f4v mma_bf16(const bf16x8& a, const bf16x8& b, f4v c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-
-// acc += A^T B over 32 rows to ~FP32 accuracy, smallest terms first.
-__device__ __forceinline__ f4v mma_bf16x3(const Split3& a, const Split3& b, f4v acc) {
-    acc = mma_bf16(a.m, b.m, acc);
-    acc = mma_bf16(a.h, b.l, acc);
-    acc = mma_bf16(a.l, b.h, acc);
-    acc = mma_bf16(a.h, b.m, acc);
-    acc = mma_bf16(a.m, b.h, acc);
-    return mma_bf16(a.h, b.h, acc);
-}
 
 // Symmetric 32 x 32 Gram accumulator of one wave: blocks (0,0), (0,1),
 // (1,1) of 16 x 16.  FP32 partials c** live in registers; flush() adds them

@@ -216,12 +216,39 @@ size_t qr_ws_bytes(int m, int n) {
 // One coupling step: part = chunk products of Q1^T A2, C = their sum (or
 // C += it), A2 -= Q1 (the new sum), as the reference's two gemm_blockqr
 // calls (block_QR.h:88-93).
+// bf16: FP32 data on the BF16 matrix cores with BF16x3 split operands
+// (gemm_tn_bf16x3 / gemm_nn_bf16x3), the mixed-precision entry points.
 template <typename T>
-int couple(int m, int n1, int n2, const T* A1, T* A2, int lda, T* C, int ldc, T* part, hipStream_t st) {
+int couple(int m, int n1, int n2, const T* A1, T* A2, int lda, T* C, int ldc, T* part, hipStream_t st,
+           bool bf16 = false) {
     const int kc = tn_chunk_rows(m, n1, n2);
     const int chunks = (m + kc - 1) / kc;
     const bool timed = timing_enabled();
     const bool vec = ((uintptr_t)A1 % 16 == 0) && (((size_t)lda * sizeof(T)) % 16 == 0);
+    if constexpr (sizeof(T) == 4) {
+        if (bf16) {
+            const dim3 gtn(chunks, (n1 + 63) / 64, (n2 + 63) / 64), gnn((m + 63) / 64, (n2 + 63) / 64);
+            if (timed) timing_mark(kPhaseGemmTN, true, st);
+            if (vec)
+                hipLaunchKernelGGL(gemm_tn_bf16x3<true>, gtn, dim3(64), 0, st, m, n1, n2, A1, (int64_t)lda,
+                                   (const T*)A2, (int64_t)lda, kc, part);
+            else
+                hipLaunchKernelGGL(gemm_tn_bf16x3<false>, gtn, dim3(64), 0, st, m, n1, n2, A1, (int64_t)lda,
+                                   (const T*)A2, (int64_t)lda, kc, part);
+            hipLaunchKernelGGL(gemm_tn_reduce<T>, dim3((n1 * n2 + 15) / 16), dim3(256), 0, st, n1, n2, chunks,
+                               (const T*)part, C, (int64_t)ldc, 0);
+            if (timed) timing_mark(kPhaseGemmTN, false, st);
+            if (timed) timing_mark(kPhaseGemmNN, true, st);
+            if (vec)
+                hipLaunchKernelGGL(gemm_nn_bf16x3<true>, gnn, dim3(64), 0, st, m, n1, n2, A1, (int64_t)lda,
+                                   (const T*)C, (int64_t)ldc, A2, (int64_t)lda);
+            else
+                hipLaunchKernelGGL(gemm_nn_bf16x3<false>, gnn, dim3(64), 0, st, m, n1, n2, A1, (int64_t)lda,
+                                   (const T*)C, (int64_t)ldc, A2, (int64_t)lda);
+            if (timed) timing_mark(kPhaseGemmNN, false, st);
+            return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
+        }
+    }
     if (timed) timing_mark(kPhaseGemmTN, true, st);
     if (vec)
         hipLaunchKernelGGL((gemm_tn_mfma<T, true>), dim3(chunks, (n1 + 63) / 64, (n2 + 63) / 64), dim3(64), 0, st, m,
@@ -244,11 +271,12 @@ int couple(int m, int n1, int n2, const T* A1, T* A2, int lda, T* C, int ldc, T*
 }
 
 // Recursive blocked QR (block_QR.h:72-95).  mixed (FP32): the panels are the
-// mixed-precision TSQR and the coupling is applied twice (block classical
-// Gram-Schmidt with reorthogonalisation, BCGS2): the second product
-// C2 = Q1^T A2 is subtracted again and added to R12, which restores the
-// orthogonality of the column blocks that one pass loses in proportion to
-// cond(A).  tmp holds n1 x n2 elements.
+// mixed-precision TSQR, the coupling products run on the BF16 matrix cores
+// with BF16x3 split operands and FP32 accumulation, and the coupling is
+// applied twice (block classical Gram-Schmidt with reorthogonalisation,
+// BCGS2): the second product C2 = Q1^T A2 is subtracted again and added to
+// R12, which restores the orthogonality of the column blocks that one pass
+// loses in proportion to cond(A).  tmp holds n1 x n2 elements.
 template <typename T>
 int qr_rec(int m, int n, T* A, int lda, T* R, int ldr, char* ws, size_t panel_bytes, T* part, T* tmp,
            hipStream_t st, bool mixed) {
@@ -261,10 +289,10 @@ int qr_rec(int m, int n, T* A, int lda, T* R, int ldr, char* ws, size_t panel_by
     T* R22 = R + (int64_t)n1 * ldr + n1;
     int e = qr_rec<T>(m, n1, A1, lda, R11, ldr, ws, panel_bytes, part, tmp, st, mixed);
     if (e) return e;
-    e = couple<T>(m, n1, n2, A1, A2, lda, R12, ldr, part, st);
+    e = couple<T>(m, n1, n2, A1, A2, lda, R12, ldr, part, st, mixed);
     if (e) return e;
     if (mixed) {
-        e = couple<T>(m, n1, n2, A1, A2, lda, tmp, n1, part, st);
+        e = couple<T>(m, n1, n2, A1, A2, lda, tmp, n1, part, st, true);
         if (e) return e;
         hipLaunchKernelGGL(add_block<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, (const T*)tmp, n1, R12,
                            (int64_t)ldr, n1, n2);
