@@ -114,13 +114,15 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
             for (int t = 0; t < d.lv[l].tiles; ++t) s.down[l].push_back(make_int2(k, t));
         }
     }
-    std::vector<int2> crow;
-    for (int c = 0; c < (int)couples.size(); ++c)
+    std::vector<int2> crow, clist;
+    for (int c = 0; c < (int)couples.size(); ++c) {
+        clist.push_back(make_int2(c, 0));
         for (int t = 0; t < couples[c].leaves; ++t) crow.push_back(make_int2(c, t));
+    }
 
     // one device allocation: problems, couples, then every item list
     size_t bytes = align256(all.size() * sizeof(BatchProb)) + align256(couples.size() * sizeof(BatchCouple));
-    size_t nitems = crow.size();
+    size_t nitems = crow.size() + clist.size();
     for (auto& s : st) {
         nitems += s.probs.size() + s.leaves.size() + s.level1.size();
         for (auto& d : s.down) nitems += d.size();
@@ -144,6 +146,7 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
     };
     auto put_items = [&](const std::vector<int2>& v) { return At{put(v.data(), v.size() * sizeof(int2)), (int)v.size()}; };
     At a_crow = put_items(crow);
+    At a_clist = put_items(clist);
     At a_probs[2], a_leaves[2], a_l1[2];
     std::vector<At> a_down[2];
     for (int s = 0; s < 2; ++s) {
@@ -167,6 +170,7 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
     dev.couples = reinterpret_cast<const BatchCouple*>(base + at_couples);
     dev.ncouples = (int)couples.size();
     dev.couple_rows = list(a_crow);
+    dev.couple_list = list(a_clist);
     for (int s = 0; s < 2; ++s) {
         dev.stage[s].fallback = s == 0 ? 0 : (int64_t)list_bytes[0];
         dev.stage[s].probs = list(a_probs[s]);

@@ -16,17 +16,19 @@
 //                   every level-1 tree block, every tile of each tree level,
 //                   every row block of every couple.
 // Each phase below is one launch over one item list, so the GPU is filled
-// by rows of all matrices at once; per-matrix steps that need the whole
-// matrix (the sum of the superleaf Grams, the residual check, the sum of the
-// coupling products) are done by the last item of the matrix to finish
-// (an atomic ticket with agent-scope release / acquire), so no wave ever
-// waits for another.  The whole sequence holds no host synchronisation and
-// no memset, so it can be captured once into a hipGraph and replayed.
+// by rows of all matrices at once; the per-matrix steps that need the whole
+// matrix (the sum of the superleaf Grams and its Cholesky, the residual
+// check, the sum of the coupling products) are one wave per matrix in a
+// launch of their own, so that the streaming launches keep their registers
+// for loads in flight and no wave ever waits for another.  The whole
+// sequence holds no host synchronisation and no memset, so it can be
+// captured once into a hipGraph and replayed.
 //
 // Phases for one panel stage (T = float or double; "mixed" is FP32 only):
 //   batch_init       tickets and control words of every problem = 0
-//   batch_mixed_gram (mixed) BF16x3 superleaf Grams; the last superleaf of
-//                    a matrix sums them and runs the flat Cholesky + gate
+//   batch_mixed_gram (mixed) BF16x3 superleaf Grams, then
+//   batch_mixed_flat one wave per matrix sums them and runs the flat
+//                    Cholesky + gate
 //   batch_super_r    Householder superleaves (every superleaf of a plain
 //                    problem; of a mixed problem only if its gate failed)
 //   batch_tree_up    the ticketed walk up each problem's tree
@@ -119,12 +121,27 @@ constexpr int batch_super_threads() {
 }
 
 // One wave adds `count` 32 x 32 FP64 blocks at src, in order, into L.G.
+// The blocks were written by other CUs, so every load goes to memory: four
+// blocks (64 loads per lane) are in flight at a time.
 __device__ __forceinline__ void sum_blocks_wave(GramLds& L, const double* __restrict__ src, int count, int lane) {
     double s[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) s[e] = 0.0;
+    int b = 0;
 #pragma unroll 1
-    for (int b = 0; b < count; ++b) {
+    for (; b + 4 <= count; b += 4) {
+        double v[4][16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[q][e] = src[(int64_t)(b + q) * 1024 + lane + 64 * e];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) s[e] += v[q][e];
+    }
+#pragma unroll 1
+    for (; b < count; ++b) {
         const double* G = src + (int64_t)b * 1024;
 #pragma unroll
         for (int e = 0; e < 16; ++e) s[e] += G[lane + 64 * e];
@@ -137,21 +154,8 @@ __device__ __forceinline__ void sum_blocks_wave(GramLds& L, const double* __rest
     wave_sync();
 }
 
-// True for the wave whose increment completes `total` arrivals on *ticket;
-// its later reads see every earlier arriver's writes.
-__device__ __forceinline__ bool last_arriver(int* ticket, int total, int lane) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    int old = 0;
-    if (lane == 0) old = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    old = __builtin_amdgcn_readfirstlane(old);
-    if (old != total - 1) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return true;
-}
-
 // ctl words of a problem (int): 0 flat CholeskyQR taken, 1 second pass
-// asked, 2 check status, 8 Gram ticket.
-constexpr int kCtlGramTicket = 8;
+// asked, 2 check status.
 
 // ---------------------------------------------------------------------------
 // Problem set-up: tickets and control words = 0; couples' tickets = 0;
@@ -306,15 +310,11 @@ __device__ __forceinline__ double* prob_side(const BatchProb& p, char* work) {
 }
 
 template <int UNUSED = 0>
-__global__ __launch_bounds__(64) void batch_mixed_gram(const BatchProb* __restrict__ probs,
-                                                        const int2* __restrict__ items, char* __restrict__ work,
-                                                        double gate, int* __restrict__ fallback) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void batch_mixed_gram(const BatchProb* __restrict__ probs,
+                                                        const int2* __restrict__ items, char* __restrict__ work) {
     __shared__ double sums[12 * 64];
-    __shared__ GramLds L;
-    const int lane = threadIdx.x;
     const int2 it = items[blockIdx.x];
     const BatchProb& p = probs[it.x];
-    char* ws = prob_ws(p, work);
     double* side = prob_side(p, work);
     const LevelDesc& L0 = p.d.lv[0];
     const int n = p.d.n;
@@ -323,19 +323,34 @@ __global__ __launch_bounds__(64) void batch_mixed_gram(const BatchProb* __restri
         mixed_gram_body<NT, decltype(VC)::value>(L0, n, reinterpret_cast<const float*>(p.A), p.lda,
                                                  side + (int64_t)it.y * 1024, sums, it.y);
     });
+}
+
+// One wave per problem, after batch_mixed_gram: the sum of its superleaf
+// Grams (in order), the flat Cholesky and the gate (a launch of its own so
+// that the streaming Gram keeps its registers).  If the gate fails, the
+// problem's superleaves go to the Householder list.
+template <int UNUSED = 0>
+__global__ __launch_bounds__(64) void batch_mixed_flat(const BatchProb* __restrict__ probs,
+                                                        const int2* __restrict__ list, char* __restrict__ work,
+                                                        double gate, int* __restrict__ fallback) {
+    __shared__ GramLds L;
+    const int lane = threadIdx.x;
+    const int pi = list[blockIdx.x].x;
+    const BatchProb& p = probs[pi];
+    char* ws = prob_ws(p, work);
+    double* side = prob_side(p, work);
+    const LevelDesc& L0 = p.d.lv[0];
     int* ctl = reinterpret_cast<int*>(ws + p.w.ctl);
-    if (!last_arriver(ctl + kCtlGramTicket, L0.tiles, lane)) return;
     sum_blocks_wave(L, side, L0.tiles, lane);
     const int64_t T0 = L0.tiles;
-    const bool ok = flat_factor<float>(L, n, reinterpret_cast<float*>(p.R), p.ldr, side + T0 * 1024, ctl, gate,
+    const bool ok = flat_factor<float>(L, p.d.n, reinterpret_cast<float*>(p.R), p.ldr, side + T0 * 1024, ctl, gate,
                                        side + (T0 + 1) * 1024, lane);
-    // gate failed: this matrix's superleaves go to the Householder list
     if (!ok) {
         int at = 0;
         if (lane == 0) at = __hip_atomic_fetch_add(fallback, L0.tiles, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         at = __builtin_amdgcn_readfirstlane(at);
         int2* dst = reinterpret_cast<int2*>(fallback + 2) + at;
-        for (int t = lane; t < L0.tiles; t += 64) dst[t] = make_int2(it.x, t);
+        for (int t = lane; t < L0.tiles; t += 64) dst[t] = make_int2(pi, t);
     }
 }
 
@@ -382,8 +397,9 @@ __global__ __launch_bounds__(64) void batch_mixed_check(const BatchProb* __restr
 
 // ---------------------------------------------------------------------------
 // Coupling of two panels: P = Q1^T A2 over a couple's rows (FP64 MFMA, FP32
-// or FP64 inputs exact in FP64), summed by the last row block into R12;
-// then A2 -= Q1 P.  `pass` 2 (mixed method) adds the second product to R12.
+// or FP64 inputs exact in FP64), summed per couple into R12
+// (batch_cross_sum); then A2 -= Q1 P.  `pass` 2 (mixed method) adds the
+// second product to R12.
 // ---------------------------------------------------------------------------
 
 // P_b = Q1_b^T A2_b for row block `blk`, written to out (FP64 32 x 32,
@@ -487,19 +503,32 @@ template <typename T>
 __global__ __launch_bounds__(64) void batch_cross(const BatchCouple* __restrict__ couples,
                                                    const int2* __restrict__ items, char* __restrict__ work,
                                                    int pass) {
-    __shared__ GramLds L;
     const int lane = threadIdx.x;
     const int2 it = items[blockIdx.x];
     const BatchCouple& q = couples[it.x];
     double* blocks = reinterpret_cast<double*>(work + q.ws);
     double* norms = reinterpret_cast<double*>(work + q.ws + couple_norms_off(q.leaves));
-    int* tk = reinterpret_cast<int*>(work + q.ws + couple_ints_off(q.leaves));
+    const int* tk = reinterpret_cast<const int*>(work + q.ws + couple_ints_off(q.leaves));
     if (pass == 2 && !tk[kCoupleReorth]) return;
     if (q.vec)
         cross_block<T, true>(q, it.y, blocks + (int64_t)it.y * 1024, norms + (int64_t)it.y * 32, lane);
     else
         cross_block<T, false>(q, it.y, blocks + (int64_t)it.y * 1024, norms + (int64_t)it.y * 32, lane);
-    if (!last_arriver(tk + pass - 1, q.leaves, lane)) return;
+}
+
+// One wave per couple, after batch_cross: P = the sum of the row blocks'
+// products (in order), R12 (+)= P.
+template <typename T>
+__global__ __launch_bounds__(64) void batch_cross_sum(const BatchCouple* __restrict__ couples,
+                                                       const int2* __restrict__ list, char* __restrict__ work,
+                                                       int pass) {
+    __shared__ GramLds L;
+    const int lane = threadIdx.x;
+    const BatchCouple& q = couples[list[blockIdx.x].x];
+    double* blocks = reinterpret_cast<double*>(work + q.ws);
+    const double* norms = reinterpret_cast<const double*>(work + q.ws + couple_norms_off(q.leaves));
+    int* tk = reinterpret_cast<int*>(work + q.ws + couple_ints_off(q.leaves));
+    if (pass == 2 && !tk[kCoupleReorth]) return;
     sum_blocks_wave(L, blocks, q.leaves, lane);
     double* P = blocks + (int64_t)q.leaves * 1024;
     double* Racc = blocks + (int64_t)(q.leaves + 1) * 1024;

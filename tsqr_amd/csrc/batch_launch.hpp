@@ -35,7 +35,8 @@ struct BatchDev {
     int nprobs = 0;
     const BatchCouple* couples = nullptr;  // device
     int ncouples = 0;
-    ItemList couple_rows;
+    ItemList couple_rows;  // (couple, row block) for every row block of every couple
+    ItemList couple_list;  // (couple, 0) for every couple
     BatchStage stage[2];  // first panels (and whole matrices with n <= 32), second panels
 };
 

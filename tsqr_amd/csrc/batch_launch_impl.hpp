@@ -18,9 +18,12 @@ void launch_panel_stage(const BatchDev& b, const BatchStage& s, char* work, hipS
     const int* hh_count = mixed ? fallback : nullptr;
     const int hh_grid = mixed ? (s.leaves.count < 1024 ? s.leaves.count : 1024) : s.leaves.count;
     if constexpr (sizeof(T) == 4) {
-        if (mixed)
+        if (mixed) {
             hipLaunchKernelGGL(batch_mixed_gram<>, dim3(s.leaves.count), dim3(64), 0, st, b.probs, s.leaves.items,
-                               work, b.gate, fallback);
+                               work);
+            hipLaunchKernelGGL(batch_mixed_flat<>, dim3(s.probs.count), dim3(64), 0, st, b.probs, s.probs.items, work,
+                               b.gate, fallback);
+        }
     }
     if (mixed)
         hipLaunchKernelGGL((batch_super_r<T, true>), dim3(hh_grid), dim3(batch_super_threads<T>()), 0, st, b.probs,
@@ -66,6 +69,8 @@ int batch_launch(const BatchDev& b, void* work_v, hipStream_t st) {
         for (int pass = 1; pass <= passes; ++pass) {
             hipLaunchKernelGGL(batch_cross<T>, dim3(b.couple_rows.count), dim3(64), 0, st, b.couples,
                                b.couple_rows.items, work, pass);
+            hipLaunchKernelGGL(batch_cross_sum<T>, dim3(b.couple_list.count), dim3(64), 0, st, b.couples,
+                               b.couple_list.items, work, pass);
             hipLaunchKernelGGL(batch_project<T>, dim3(b.couple_rows.count), dim3(64), 0, st, b.couples,
                                b.couple_rows.items, work, pass);
         }

@@ -110,7 +110,7 @@ __device__ __forceinline__ void load8(float (&x)[8], const float* __restrict__ b
 // Pass 1a: G_b = A_b^T A_b of every superleaf (one wave each) with BF16x3
 // products, written to side[leaf] (FP64, row-major 32 x 32).  Lane l loads
 // rows 8(l/16)..+7 of column l%16 (and 16 + l%16) of each 32-row step; the
-// four steps of a 128-row batch are loaded together and the next batch is in
+// two steps of a 64-row batch are loaded together and the next batch is in
 // flight while the current one is multiplied.
 template <int NT, bool VEC>
 __device__ __forceinline__ void mixed_gram_body(const LevelDesc& L0, int n, const float* __restrict__ A, int lda,
@@ -120,7 +120,7 @@ __device__ __forceinline__ void mixed_gram_body(const LevelDesc& L0, int n, cons
     const float* base = A + (int64_t)leaf * L0.tile_rows;
     const int col = lane & 15;
     const int rq = 8 * (lane >> 4);
-    constexpr int D = 4;  // 32-row steps per batch
+    constexpr int D = 2;  // 32-row steps per batch (a batch and its successor in flight: 64 VGPRs per column half)
     GramAcc32 g;
     g.sums = sums;
     g.zero(lane);
@@ -159,7 +159,7 @@ __device__ __forceinline__ void mixed_gram_body(const LevelDesc& L0, int n, cons
 }
 
 template <int NT, bool VEC>
-__global__ __launch_bounds__(64) void tsqr_mixed_gram(const TreeDesc d, const float* __restrict__ A, int lda,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void tsqr_mixed_gram(const TreeDesc d, const float* __restrict__ A, int lda,
                                                       double* __restrict__ side) {
     __shared__ double sums[12 * 64];
     mixed_gram_body<NT, VEC>(d.lv[0], d.n, A, lda, side + (int64_t)blockIdx.x * 1024, sums, blockIdx.x);
