@@ -82,3 +82,25 @@ def test_check_rejects_bad_arguments():
     assert L.tsqr_check_s(10, 33, 1, 10, 1, 10, 1, 33, norms, None) != 0  # n > 32
     assert L.tsqr_check_s(10, 4, None, 10, 1, 10, 1, 4, norms, None) != 0  # null A
     assert L.tsqr_check_d(10, 4, 1, 9, 1, 10, 1, 4, norms, None) != 0  # lda < m
+
+
+def test_torch_wrappers_r_only_and_check():
+    """tsqr_amd.tsqr_r leaves A alone and matches tsqr's R; tsqr_amd.check
+    on tsqr's output agrees with the host metrics."""
+    import torch
+
+    import tsqr_amd
+
+    A_h = np.random.default_rng(17).standard_normal((40000, 32)).astype(np.float32)
+    A = tsqr_amd.colmajor(torch.from_numpy(A_h).cuda())
+    keep = A.clone()
+    R_only = tsqr_amd.tsqr_r(A)
+    torch.cuda.synchronize()
+    assert torch.equal(A, keep)
+    Q, R = tsqr_amd.tsqr(A)
+    torch.cuda.synchronize()
+    assert torch.equal(R_only, R)
+    dev = tsqr_amd.check(keep, Q, R)
+    host = O.metrics(A_h, Q.cpu().numpy(), R.cpu().numpy())
+    _agree(dev["backward_fro"], host["backward"], "backward_fro")
+    _agree(dev["orth_fro"], host["orth"], "orth_fro")

@@ -11,6 +11,9 @@ that takes torch tensors only as holders of device memory and streams:
                                  float32 TSQR on the BF16 matrix cores
                                  (CholeskyQR with a residual check and an
                                  optional second pass), same convention
+    tsqr_r(A, R=None, work=None) R only; A is read, never written
+    check(A, Q, R)               device-side FP64 backward and
+                                 orthogonality errors of A = QR
     Batch(mats, method="plain"|"mixed")
                                  many uneven TSQRs (n <= 64) planned once,
                                  run as a few launches or one hipGraph
@@ -31,11 +34,12 @@ from ._lib import (  # noqa: F401
     qr_workspace_bytes,
     tsqr_workspace_bytes,
 )
-from .api import Batch, colmajor, empty_colmajor, qr, qr_mixed, tsqr, tsqr_mixed  # noqa: F401
+from .api import Batch, check, colmajor, empty_colmajor, qr, qr_mixed, tsqr, tsqr_mixed, tsqr_r  # noqa: F401
 
 __all__ = [
     "Batch",
     "LIB_PATH",
+    "check",
     "TsqrError",
     "colmajor",
     "empty_colmajor",
@@ -45,5 +49,6 @@ __all__ = [
     "qr_workspace_bytes",
     "tsqr",
     "tsqr_mixed",
+    "tsqr_r",
     "tsqr_workspace_bytes",
 ]

@@ -36,7 +36,7 @@ def _run(kind: str, A: torch.Tensor, R, work, stream):
     if R is None:
         R = empty_colmajor(n, n, A.dtype, A.device)
     ldr = _ld(R, "R")
-    need = (qr_workspace_bytes if kind.startswith("qr") else tsqr_workspace_bytes)(m, n, code)
+    need = (qr_workspace_bytes if kind.startswith("qr") else tsqr_workspace_bytes)(m, n, code)  # tsqr_r: as tsqr
     if work is None:
         work = torch.empty(max(need, 1), dtype=torch.uint8, device=A.device)
     nbytes = work.numel() * work.element_size()
@@ -47,7 +47,10 @@ def _run(kind: str, A: torch.Tensor, R, work, stream):
         if code != TSQR_FLOAT:
             raise TypeError("the mixed-precision TSQR takes float32 input")
         suf = "s"
-    fn = getattr(lib(), f"{kind}_{suf}_stream")
+    if kind == "tsqr_r":
+        fn = getattr(lib(), f"tsqr_r_{suf}_stream")
+    else:
+        fn = getattr(lib(), f"{kind}_{suf}_stream")
     st = fn(m, n, A.data_ptr(), lda, R.data_ptr(), ldr, work.data_ptr(), nbytes, stream.cuda_stream)
     if st != 0:
         raise TsqrError(st, f"{kind}_{suf}_stream(m={m}, n={n})")
@@ -58,6 +61,28 @@ def tsqr(A: torch.Tensor, R: torch.Tensor | None = None, work: torch.Tensor | No
     """TSQR of a column-major m x n GPU tensor, n <= 32 (TallShinnyQR.h:86-149).
     A is overwritten by the explicit thin Q; returns (A, R)."""
     return _run("tsqr", A, R, work, stream)
+
+
+def tsqr_r(A: torch.Tensor, R: torch.Tensor | None = None, work: torch.Tensor | None = None, stream=None):
+    """R factor only of a column-major m x n GPU tensor, n <= 32
+    (tsqr_r_s / tsqr_r_d): A is read and left unchanged; returns R."""
+    return _run("tsqr_r", A, R, work, stream)[1]
+
+
+def check(A: torch.Tensor, Q: torch.Tensor, R: torch.Tensor, stream=None) -> dict:
+    """Device-side FP64 check of A = QR (tsqr_check_s / tsqr_check_d, n <= 32):
+    backward_fro = ||A - QR||_F / ||A||_F, orth_fro = ||I - Q^T Q||_F and
+    their spectral-norm counterparts backward_2, orth_2.  Waits for the
+    result."""
+    from ._lib import check as _check
+
+    m, n = A.shape
+    if Q.shape != (m, n) or R.shape != (n, n) or Q.dtype != A.dtype or R.dtype != A.dtype:
+        raise ValueError("A, Q must be m x n and R n x n, all of one dtype")
+    if stream is None:
+        stream = torch.cuda.current_stream(A.device)
+    return _check(m, n, _code(A.dtype), A.data_ptr(), _ld(A, "A"), Q.data_ptr(), _ld(Q, "Q"), R.data_ptr(),
+                  _ld(R, "R"), stream.cuda_stream)
 
 
 def qr(A: torch.Tensor, R: torch.Tensor | None = None, work: torch.Tensor | None = None, stream=None):
