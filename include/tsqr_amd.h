@@ -238,6 +238,28 @@ int tsqr_r_s_stream(int m, int n, const float *A, int lda, float *R, int ldr, vo
 int tsqr_r_d_stream(int m, int n, const double *A, int lda, double *R, int ldr, void *work, size_t work_bytes,
                     void *stream);
 
+/* Least squares and Q^T B on the implicit Q of the R-only mode (no
+ * counterpart in the reference; the most common use of a tall-skinny R).
+ * n <= 32, m >= n, A of full column rank; every pointer on the device.
+ *
+ * tsqr_lstsq_*: X (n x nrhs, ldx) = argmin ||A X - B||_F by the corrected
+ * seminormal equations (Bjorck): R from tsqr_r_*, X = R^-1 R^-T A^T B, then
+ * one refinement step with the residual.  A is read only; B (m x nrhs) is
+ * overwritten by the final residual B - A X.
+ * tsqr_apply_qt_*: C (n x k, ldc) = Q^T B with Q = A R^-1 (R from tsqr_r_*
+ * on the same A).
+ * Both take work of tsqr_solve_workspace_bytes(m, n, nrhs or k, dtype)
+ * bytes and run on `stream`. */
+size_t tsqr_solve_workspace_bytes(int m, int n, int nrhs, int dtype);
+int tsqr_lstsq_s_stream(int m, int n, int nrhs, const float *A, int lda, float *B, int ldb, float *X, int ldx,
+                        void *work, size_t work_bytes, void *stream);
+int tsqr_lstsq_d_stream(int m, int n, int nrhs, const double *A, int lda, double *B, int ldb, double *X, int ldx,
+                        void *work, size_t work_bytes, void *stream);
+int tsqr_apply_qt_s_stream(int m, int n, int k, const float *A, int lda, const float *R, int ldr, const float *B,
+                           int ldb, float *C, int ldc, void *work, size_t work_bytes, void *stream);
+int tsqr_apply_qt_d_stream(int m, int n, int k, const double *A, int lda, const double *R, int ldr,
+                           const double *B, int ldb, double *C, int ldc, void *work, size_t work_bytes, void *stream);
+
 /* Device-side FP64 checker of a factorisation, replaces check_QR_accuracy
  * (utils.h:317-455), whose m x m gesvd workspace limits it to m <= ~46k.
  * A is the input (m x n, lda), Q and R the computed factors (ldq, ldr),

@@ -56,6 +56,11 @@ EXPORTS = (
     "tsqr_r_d",
     "tsqr_r_s_stream",
     "tsqr_r_d_stream",
+    "tsqr_solve_workspace_bytes",
+    "tsqr_lstsq_s_stream",
+    "tsqr_lstsq_d_stream",
+    "tsqr_apply_qt_s_stream",
+    "tsqr_apply_qt_d_stream",
 )
 
 
@@ -128,6 +133,16 @@ def lib() -> ctypes.CDLL:
     for name in ("tsqr_check_s", "tsqr_check_d"):
         f = getattr(L, name)
         f.argtypes = [i32, i32, vp, i32, vp, i32, vp, i32, ctypes.POINTER(ctypes.c_double), vp]
+        f.restype = i32
+    L.tsqr_solve_workspace_bytes.argtypes = [i32, i32, i32, i32]
+    L.tsqr_solve_workspace_bytes.restype = sz
+    for name in ("tsqr_lstsq_s_stream", "tsqr_lstsq_d_stream"):
+        f = getattr(L, name)
+        f.argtypes = [i32, i32, i32, vp, i32, vp, i32, vp, i32, vp, sz, vp]
+        f.restype = i32
+    for name in ("tsqr_apply_qt_s_stream", "tsqr_apply_qt_d_stream"):
+        f = getattr(L, name)
+        f.argtypes = [i32, i32, i32, vp, i32, vp, i32, vp, i32, vp, i32, vp, sz, vp]
         f.restype = i32
     L.tsqr_timing_enable.argtypes = [i32]
     L.tsqr_timing_enable.restype = i32

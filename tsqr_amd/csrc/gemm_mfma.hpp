@@ -372,6 +372,59 @@ __global__ __launch_bounds__(64) void gemm_nn_bf16x3(int m, int K, int N, const 
         }
 }
 
+// ---------------------------------------------------------------------------
+// Small triangular solves of the least-squares entry points (n <= 32): for
+// column j < k of C (n x k, ldc), y = R^{-T} C[:, j] (forward substitution
+// with the upper triangular R, n x n, ldr) and, unless transpose_only,
+// x = R^{-1} y (back substitution), in FP64, one thread per column.  The
+// result is written to X[:, j] (ldx), or added to it if accumulate.
+// R is staged in LDS; the loops run to 32 with guards so the vectors stay
+// in registers.
+template <typename T>
+__global__ __launch_bounds__(256) void tri_normal_solve(int n, int k, const T* __restrict__ R, int64_t ldr,
+                                                        const T* __restrict__ C, int64_t ldc, T* __restrict__ X,
+                                                        int64_t ldx, int accumulate, int transpose_only) {
+    __shared__ double Rs[32][33];
+    for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+        const int i = e & 31, j = e >> 5;
+        Rs[i][j] = (i < n && j < n && i <= j) ? (double)R[i + (int64_t)j * ldr] : 0.0;
+    }
+    __syncthreads();
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= k) return;
+    double y[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) y[i] = i < n ? (double)C[i + (int64_t)col * ldc] : 0.0;
+    // R^T y' = y: row i of R^T is column i of R
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+        if (i < n) {
+            double s = y[i];
+#pragma unroll
+            for (int p = 0; p < i; ++p) s -= Rs[p][i] * y[p];
+            y[i] = s / Rs[i][i];
+        }
+    }
+    if (!transpose_only) {
+#pragma unroll
+        for (int i = 31; i >= 0; --i) {
+            if (i < n) {
+                double s = y[i];
+#pragma unroll
+                for (int p = i + 1; p < 32; ++p) s -= Rs[i][p] * y[p];
+                y[i] = s / Rs[i][i];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+        if (i < n) {
+            T* x = X + i + (int64_t)col * ldx;
+            *x = accumulate ? (T)((double)*x + y[i]) : (T)y[i];
+        }
+    }
+}
+
 // Zeroes rows [r0, r0+nr) x columns [c0, c0+nc) of a column-major matrix.
 template <typename T>
 __global__ void zero_block(T* __restrict__ M, int64_t ld, int r0, int nr, int c0, int nc) {

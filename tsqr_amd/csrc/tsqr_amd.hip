@@ -402,6 +402,122 @@ int check_impl(int m, int n, const T* A, int lda, const T* Q, int ldq, const T* 
     return TSQR_OK;
 }
 
+// ------------------------------------------------------------------------
+// Least squares and Q^T B on the implicit Q = A R^{-1} (R-only mode).
+// ------------------------------------------------------------------------
+
+// C (n1 x n2, ldc) = X^T Y over m rows: split-K MFMA products and their
+// ordered sum (the coupling kernels of qr_rec).
+template <typename T>
+int tn_product(int m, int n1, int n2, const T* X, int ldx, const T* Y, int ldy, T* C, int ldc, T* part,
+               hipStream_t st) {
+    const int kc = tn_chunk_rows(m, n1, n2);
+    const int chunks = (m + kc - 1) / kc;
+    const dim3 grid(chunks, (n1 + 63) / 64, (n2 + 63) / 64);
+    if (aligned16(X, ldx, sizeof(T)) && aligned16(Y, ldy, sizeof(T)))
+        hipLaunchKernelGGL((gemm_tn_mfma<T, true>), grid, dim3(64), 0, st, m, n1, n2, X, (int64_t)ldx, Y,
+                           (int64_t)ldy, kc, part);
+    else
+        hipLaunchKernelGGL((gemm_tn_mfma<T, false>), grid, dim3(64), 0, st, m, n1, n2, X, (int64_t)ldx, Y,
+                           (int64_t)ldy, kc, part);
+    hipLaunchKernelGGL(gemm_tn_reduce<T>, dim3((n1 * n2 + 15) / 16), dim3(256), 0, st, n1, n2, chunks,
+                       (const T*)part, C, (int64_t)ldc, 0);
+    return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
+}
+
+// C (m x N, ldc) -= X (m x K, ldx) B (K x N, ldb).
+template <typename T>
+int nn_update(int m, int K, int N, const T* X, int ldx, const T* B, int ldb, T* C, int ldc, hipStream_t st) {
+    const dim3 grid((m + 63) / 64, (N + 63) / 64);
+    if (aligned16(X, ldx, sizeof(T)) && aligned16(C, ldc, sizeof(T)))
+        hipLaunchKernelGGL((gemm_nn_mfma<T, true>), grid, dim3(64), 0, st, m, K, N, X, (int64_t)ldx, B,
+                           (int64_t)ldb, C, (int64_t)ldc);
+    else
+        hipLaunchKernelGGL((gemm_nn_mfma<T, false>), grid, dim3(64), 0, st, m, K, N, X, (int64_t)ldx, B,
+                           (int64_t)ldb, C, (int64_t)ldc);
+    return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
+}
+
+// Workspace of tsqr_lstsq_* / tsqr_apply_qt_*: the R-only TSQR's, then R
+// (n x n), the split-K partials of an n x nrhs product, and two n x nrhs
+// blocks.
+struct SolveLayout {
+    size_t tsqr, R, part, C, D, total;
+};
+template <typename T>
+SolveLayout solve_layout(int m, int n, int nrhs) {
+    SolveLayout L;
+    L.tsqr = 0;
+    L.R = align256(tsqr_ws_bytes<T>(m, n));
+    const int kc = tn_chunk_rows(m, n, nrhs);
+    const size_t chunks = (size_t)((m + kc - 1) / kc);
+    L.part = L.R + align256((size_t)n * n * sizeof(T));
+    L.C = L.part + align256(chunks * n * nrhs * sizeof(T));
+    L.D = L.C + align256((size_t)n * nrhs * sizeof(T));
+    L.total = L.D + align256((size_t)n * nrhs * sizeof(T));
+    return L;
+}
+
+template <typename T>
+int solve_args(int m, int n, int nrhs, const T* A, int lda, const void* B, int ldb, const void* X, int ldx) {
+    if (!A || !B || !X || n < 1 || m < n || nrhs < 1 || lda < m || ldb < m || ldx < n) return TSQR_ERR_ARG;
+    if (n > 32) return TSQR_ERR_SHAPE;
+    return TSQR_OK;
+}
+
+// min ||A X - B||_F by the corrected seminormal equations (Bjorck): R from
+// the R-only TSQR, X = R^{-1} R^{-T} A^T B, then one refinement step with
+// the residual E = B - A X: X += R^{-1} R^{-T} A^T E.  B is overwritten by
+// the final residual.
+template <typename T>
+int lstsq_stream(int m, int n, int nrhs, const T* A, int lda, T* B, int ldb, T* X, int ldx, void* work,
+                 size_t work_bytes, hipStream_t st) {
+    int e = solve_args<T>(m, n, nrhs, A, lda, B, ldb, X, ldx);
+    if (e) return e;
+    const SolveLayout L = solve_layout<T>(m, n, nrhs);
+    if (!work || work_bytes < L.total) return TSQR_ERR_WORKSPACE;
+    char* w = reinterpret_cast<char*>(work);
+    T* R = reinterpret_cast<T*>(w + L.R);
+    T* part = reinterpret_cast<T*>(w + L.part);
+    T* C = reinterpret_cast<T*>(w + L.C);
+    T* D = reinterpret_cast<T*>(w + L.D);
+    const dim3 tgrid((nrhs + 255) / 256);
+    e = tsqr_core<T>(m, n, const_cast<T*>(A), lda, R, n, w + L.tsqr, st, false, true);
+    if (e) return e;
+    e = tn_product<T>(m, n, nrhs, A, lda, B, ldb, C, n, part, st);  // C = A^T B
+    if (e) return e;
+    hipLaunchKernelGGL(tri_normal_solve<T>, tgrid, dim3(256), 0, st, n, nrhs, (const T*)R, (int64_t)n, (const T*)C,
+                       (int64_t)n, X, (int64_t)ldx, 0, 0);
+    e = nn_update<T>(m, n, nrhs, A, lda, X, ldx, B, ldb, st);  // B = E = B - A X
+    if (e) return e;
+    e = tn_product<T>(m, n, nrhs, A, lda, B, ldb, C, n, part, st);  // C = A^T E
+    if (e) return e;
+    hipLaunchKernelGGL(tri_normal_solve<T>, tgrid, dim3(256), 0, st, n, nrhs, (const T*)R, (int64_t)n, (const T*)C,
+                       (int64_t)n, D, (int64_t)n, 0, 0);  // D = correction
+    hipLaunchKernelGGL(add_block<T>, dim3((n * nrhs + 255) / 256), dim3(256), 0, st, (const T*)D, n, X,
+                       (int64_t)ldx, n, nrhs);
+    return nn_update<T>(m, n, nrhs, A, lda, D, n, B, ldb, st);  // B = B - A D
+}
+
+// C (n x k) = Q^T B for the implicit Q = A R^{-1} of tsqr_r_* (R given).
+template <typename T>
+int apply_qt_stream(int m, int n, int k, const T* A, int lda, const T* R, int ldr, const T* B, int ldb, T* C, int ldc,
+                    void* work, size_t work_bytes, hipStream_t st) {
+    int e = solve_args<T>(m, n, k, A, lda, B, ldb, C, ldc);
+    if (e) return e;
+    if (!R || ldr < n) return TSQR_ERR_ARG;
+    const SolveLayout L = solve_layout<T>(m, n, k);
+    if (!work || work_bytes < L.total) return TSQR_ERR_WORKSPACE;
+    char* w = reinterpret_cast<char*>(work);
+    T* part = reinterpret_cast<T*>(w + L.part);
+    T* Cw = reinterpret_cast<T*>(w + L.C);
+    e = tn_product<T>(m, n, k, A, lda, B, ldb, Cw, n, part, st);  // A^T B
+    if (e) return e;
+    hipLaunchKernelGGL(tri_normal_solve<T>, dim3((k + 255) / 256), dim3(256), 0, st, n, k, R, (int64_t)ldr,
+                       (const T*)Cw, (int64_t)n, C, (int64_t)ldc, 0, 1);
+    return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
+}
+
 template <typename T>
 int with_ref_workspace(int m, int n, T* work, int ldwork, size_t need, void** ws, size_t* bytes) {
     const size_t have = (work && ldwork > 0) ? (size_t)ldwork * (size_t)n * sizeof(T) : 0;
@@ -461,6 +577,30 @@ int tsqr_check_s(int m, int n, const float* A, int lda, const float* Q, int ldq,
 int tsqr_check_d(int m, int n, const double* A, int lda, const double* Q, int ldq, const double* R, int ldr,
                  double* norms, void* stream) {
     return check_impl<double>(m, n, A, lda, Q, ldq, R, ldr, norms, (hipStream_t)stream);
+}
+
+size_t tsqr_solve_workspace_bytes(int m, int n, int nrhs, int dtype) {
+    if (n < 1 || n > 32 || m < n || nrhs < 1) return 0;
+    if (dtype == TSQR_FLOAT) return solve_layout<float>(m, n, nrhs).total;
+    if (dtype == TSQR_DOUBLE) return solve_layout<double>(m, n, nrhs).total;
+    return 0;
+}
+
+int tsqr_lstsq_s_stream(int m, int n, int nrhs, const float* A, int lda, float* B, int ldb, float* X, int ldx,
+                        void* work, size_t work_bytes, void* stream) {
+    return lstsq_stream<float>(m, n, nrhs, A, lda, B, ldb, X, ldx, work, work_bytes, (hipStream_t)stream);
+}
+int tsqr_lstsq_d_stream(int m, int n, int nrhs, const double* A, int lda, double* B, int ldb, double* X, int ldx,
+                        void* work, size_t work_bytes, void* stream) {
+    return lstsq_stream<double>(m, n, nrhs, A, lda, B, ldb, X, ldx, work, work_bytes, (hipStream_t)stream);
+}
+int tsqr_apply_qt_s_stream(int m, int n, int k, const float* A, int lda, const float* R, int ldr, const float* B,
+                           int ldb, float* C, int ldc, void* work, size_t work_bytes, void* stream) {
+    return apply_qt_stream<float>(m, n, k, A, lda, R, ldr, B, ldb, C, ldc, work, work_bytes, (hipStream_t)stream);
+}
+int tsqr_apply_qt_d_stream(int m, int n, int k, const double* A, int lda, const double* R, int ldr, const double* B,
+                           int ldb, double* C, int ldc, void* work, size_t work_bytes, void* stream) {
+    return apply_qt_stream<double>(m, n, k, A, lda, R, ldr, B, ldb, C, ldc, work, work_bytes, (hipStream_t)stream);
 }
 
 int tsqr_timing_enable(int enable) {
