@@ -120,16 +120,22 @@ constexpr int batch_super_threads() {
     return 64 * (a > b ? (a > c ? a : c) : (b > c ? b : c));
 }
 
-// One wave adds `count` 32 x 32 FP64 blocks at src, in order, into L.G.
-// The blocks were written by other CUs, so every load goes to memory: four
-// blocks (64 loads per lane) are in flight at a time.
-__device__ __forceinline__ void sum_blocks_wave(GramLds& L, const double* __restrict__ src, int count, int lane) {
+// L.G = the sum of `count` 32 x 32 FP64 blocks at src, by a workgroup of
+// four waves (256 threads): wave w adds the blocks [w c / 4, (w + 1) c / 4)
+// in order, four blocks in flight (they were written by other CUs, so every
+// load goes to memory), then the four wave sums are added in wave order in
+// LDS, so the result is fixed for a given count.  Every thread of the
+// workgroup must call it.
+__device__ __forceinline__ void sum_blocks_wg4(GramLds& L, const double* __restrict__ src, int count) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int b0 = (int)((int64_t)count * w / 4), b1 = (int)((int64_t)count * (w + 1) / 4);
     double s[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) s[e] = 0.0;
-    int b = 0;
+    int b = b0;
 #pragma unroll 1
-    for (; b + 4 <= count; b += 4) {
+    for (; b + 4 <= b1; b += 4) {
         double v[4][16];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -141,17 +147,20 @@ __device__ __forceinline__ void sum_blocks_wave(GramLds& L, const double* __rest
             for (int e = 0; e < 16; ++e) s[e] += v[q][e];
     }
 #pragma unroll 1
-    for (; b < count; ++b) {
-        const double* G = src + (int64_t)b * 1024;
+    for (; b < b1; ++b)
 #pragma unroll
-        for (int e = 0; e < 16; ++e) s[e] += G[lane + 64 * e];
-    }
+        for (int e = 0; e < 16; ++e) s[e] += src[(int64_t)b * 1024 + lane + 64 * e];
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        if (w == k) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        const int idx = lane + 64 * e;
-        L.G[idx >> 5][idx & 31] = s[e];
+            for (int e = 0; e < 16; ++e) {
+                const int idx = lane + 64 * e;
+                L.G[idx >> 5][idx & 31] = (k == 0 ? 0.0 : L.G[idx >> 5][idx & 31]) + s[e];
+            }
+        }
+        __syncthreads();
     }
-    wave_sync();
 }
 
 // ctl words of a problem (int): 0 flat CholeskyQR taken, 1 second pass
@@ -325,14 +334,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ba
     });
 }
 
-// One wave per problem, after batch_mixed_gram: the sum of its superleaf
-// Grams (in order), the flat Cholesky and the gate (a launch of its own so
-// that the streaming Gram keeps its registers).  If the gate fails, the
+// One workgroup per problem, after batch_mixed_gram: the sum of its
+// superleaf Grams (four waves, fixed order), then by one wave the flat
+// Cholesky and the gate (a launch of its own so that the streaming Gram
+// keeps its registers).  If the gate fails, the
 // problem's superleaves go to the Householder list.
 template <int UNUSED = 0>
-__global__ __launch_bounds__(64) void batch_mixed_flat(const BatchProb* __restrict__ probs,
-                                                        const int2* __restrict__ list, char* __restrict__ work,
-                                                        double gate, int* __restrict__ fallback) {
+__global__ __launch_bounds__(256) void batch_mixed_flat(const BatchProb* __restrict__ probs,
+                                                         const int2* __restrict__ list, char* __restrict__ work,
+                                                         double gate, int* __restrict__ fallback) {
     __shared__ GramLds L;
     const int lane = threadIdx.x;
     const int pi = list[blockIdx.x].x;
@@ -341,7 +351,8 @@ __global__ __launch_bounds__(64) void batch_mixed_flat(const BatchProb* __restri
     double* side = prob_side(p, work);
     const LevelDesc& L0 = p.d.lv[0];
     int* ctl = reinterpret_cast<int*>(ws + p.w.ctl);
-    sum_blocks_wave(L, side, L0.tiles, lane);
+    sum_blocks_wg4(L, side, L0.tiles);
+    if (threadIdx.x >= 64) return;
     const int64_t T0 = L0.tiles;
     const bool ok = flat_factor<float>(L, p.d.n, reinterpret_cast<float*>(p.R), p.ldr, side + T0 * 1024, ctl, gate,
                                        side + (T0 + 1) * 1024, lane);
@@ -376,12 +387,13 @@ __global__ __launch_bounds__(64, 2) void batch_mixed_q(const BatchProb* __restri
     });
 }
 
-// Residual check of one problem after a Q pass (one wave per problem; a
-// launch of its own so that the Q pass keeps its registers for streaming).
+// Residual check of one problem after a Q pass (one workgroup per problem:
+// four waves sum the superleaf Grams, one runs the check; a launch of its
+// own so that the Q pass keeps its registers for streaming).
 template <int UNUSED = 0>
-__global__ __launch_bounds__(64) void batch_mixed_check(const BatchProb* __restrict__ probs,
-                                                         const int2* __restrict__ list, char* __restrict__ work,
-                                                         int pass, double tol) {
+__global__ __launch_bounds__(256) void batch_mixed_check(const BatchProb* __restrict__ probs,
+                                                          const int2* __restrict__ list, char* __restrict__ work,
+                                                          int pass, double tol) {
     __shared__ GramLds L;
     const int lane = threadIdx.x;
     const BatchProb& p = probs[list[blockIdx.x].x];
@@ -390,7 +402,8 @@ __global__ __launch_bounds__(64) void batch_mixed_check(const BatchProb* __restr
     if (!p.mixed || !ctl[0] || (pass == 2 && !ctl[1])) return;
     double* side = prob_side(p, work);
     const int64_t T0 = p.d.lv[0].tiles;
-    sum_blocks_wave(L, side, (int)T0, lane);
+    sum_blocks_wg4(L, side, (int)T0);
+    if (threadIdx.x >= 64) return;
     mixed_check_body<float>(L, p.d.n, reinterpret_cast<float*>(p.R), p.ldr, side + T0 * 1024,
                             side + (T0 + 1) * 1024, side + (T0 + 2) * 1024, ctl, pass, tol, lane);
 }
@@ -431,7 +444,7 @@ __device__ __forceinline__ void cross_block(const BatchCouple& q, int blk, doubl
             c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y1, c11, 0, 0, 0);
         }
     };
-    constexpr int D = 4;
+    constexpr int D = 4;  // 16-row slabs per batch
     int s0 = 0;
     if (VEC && rows >= D * 16) {
         Raw4<T> cq0[D], cq1[D], ca0[D], ca1[D];
@@ -516,12 +529,12 @@ __global__ __launch_bounds__(64) void batch_cross(const BatchCouple* __restrict_
         cross_block<T, false>(q, it.y, blocks + (int64_t)it.y * 1024, norms + (int64_t)it.y * 32, lane);
 }
 
-// One wave per couple, after batch_cross: P = the sum of the row blocks'
-// products (in order), R12 (+)= P.
+// One workgroup per couple, after batch_cross: P = the sum of the row
+// blocks' products (four waves, fixed order), R12 (+)= P by one wave.
 template <typename T>
-__global__ __launch_bounds__(64) void batch_cross_sum(const BatchCouple* __restrict__ couples,
-                                                       const int2* __restrict__ list, char* __restrict__ work,
-                                                       int pass) {
+__global__ __launch_bounds__(256) void batch_cross_sum(const BatchCouple* __restrict__ couples,
+                                                        const int2* __restrict__ list, char* __restrict__ work,
+                                                        int pass) {
     __shared__ GramLds L;
     const int lane = threadIdx.x;
     const BatchCouple& q = couples[list[blockIdx.x].x];
@@ -529,7 +542,8 @@ __global__ __launch_bounds__(64) void batch_cross_sum(const BatchCouple* __restr
     const double* norms = reinterpret_cast<const double*>(work + q.ws + couple_norms_off(q.leaves));
     int* tk = reinterpret_cast<int*>(work + q.ws + couple_ints_off(q.leaves));
     if (pass == 2 && !tk[kCoupleReorth]) return;
-    sum_blocks_wave(L, blocks, q.leaves, lane);
+    sum_blocks_wg4(L, blocks, q.leaves);
+    if (threadIdx.x >= 64) return;
     double* P = blocks + (int64_t)q.leaves * 1024;
     double* Racc = blocks + (int64_t)(q.leaves + 1) * 1024;
     T* R12 = reinterpret_cast<T*>(q.R12);

@@ -21,8 +21,8 @@ void launch_panel_stage(const BatchDev& b, const BatchStage& s, char* work, hipS
         if (mixed) {
             hipLaunchKernelGGL(batch_mixed_gram<>, dim3(s.leaves.count), dim3(64), 0, st, b.probs, s.leaves.items,
                                work);
-            hipLaunchKernelGGL(batch_mixed_flat<>, dim3(s.probs.count), dim3(64), 0, st, b.probs, s.probs.items, work,
-                               b.gate, fallback);
+            hipLaunchKernelGGL(batch_mixed_flat<>, dim3(s.probs.count), dim3(256), 0, st, b.probs, s.probs.items,
+                               work, b.gate, fallback);
         }
     }
     if (mixed)
@@ -42,8 +42,8 @@ void launch_panel_stage(const BatchDev& b, const BatchStage& s, char* work, hipS
             for (int pass = 1; pass <= 2; ++pass) {
                 hipLaunchKernelGGL(batch_mixed_q<>, dim3(s.leaves.count), dim3(64), 0, st, b.probs, s.leaves.items,
                                    work, pass);
-                hipLaunchKernelGGL(batch_mixed_check<>, dim3(s.probs.count), dim3(64), 0, st, b.probs, s.probs.items,
-                                   work, pass, b.tol);
+                hipLaunchKernelGGL(batch_mixed_check<>, dim3(s.probs.count), dim3(256), 0, st, b.probs,
+                                   s.probs.items, work, pass, b.tol);
             }
     }
     if (mixed)
@@ -69,7 +69,7 @@ int batch_launch(const BatchDev& b, void* work_v, hipStream_t st) {
         for (int pass = 1; pass <= passes; ++pass) {
             hipLaunchKernelGGL(batch_cross<T>, dim3(b.couple_rows.count), dim3(64), 0, st, b.couples,
                                b.couple_rows.items, work, pass);
-            hipLaunchKernelGGL(batch_cross_sum<T>, dim3(b.couple_list.count), dim3(64), 0, st, b.couples,
+            hipLaunchKernelGGL(batch_cross_sum<T>, dim3(b.couple_list.count), dim3(256), 0, st, b.couples,
                                b.couple_list.items, work, pass);
             hipLaunchKernelGGL(batch_project<T>, dim3(b.couple_rows.count), dim3(64), 0, st, b.couples,
                                b.couple_rows.items, work, pass);

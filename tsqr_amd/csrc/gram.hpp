@@ -491,11 +491,36 @@ __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restric
     }
 }
 
+// Sums `count` 32 x 32 FP64 blocks at src in index order (256 threads):
+// out[e] is entry t + 256 e of the sum.  Eight blocks are in flight per
+// step (the blocks come from other CUs, so every load goes to memory).
+__device__ __forceinline__ void sum_blocks(const double* __restrict__ src, int count, int t, double (&out)[4]) {
+    double s[4] = {0, 0, 0, 0};
+    int b = 0;
+#pragma unroll 1
+    for (; b + 8 <= count; b += 8) {
+        double v[8][4];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[q][e] = src[(int64_t)(b + q) * 1024 + t + 256 * e];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) s[e] += v[q][e];
+    }
+    for (; b < count; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s[e] += src[(int64_t)b * 1024 + t + 256 * e];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) out[e] = s[e];
+}
+
 // Flat Gram path, stage 1: partial[p] = sum of the 32 x 32 blocks
 // [p*c, (p+1)*c) of side, c = ceil(leaves / gridDim.x), added in order
-// (deterministic).  256 threads, 4 entries each.  Does nothing if run_if is
-// given and run_if[0] == 0.  (A template only so that every translation
-// unit that includes this header gets its own copy.)
+// (deterministic).  256 threads, 4 entries each, eight blocks in flight.
+// Does nothing if run_if is given and run_if[0] == 0.  (A template only so
+// that every translation unit that includes this header gets its own copy.)
 template <int UNUSED = 0>
 __global__ __launch_bounds__(256) void tsqr_gram_partial(int leaves, const double* __restrict__ side,
                                                          double* __restrict__ partial,
@@ -504,13 +529,8 @@ __global__ __launch_bounds__(256) void tsqr_gram_partial(int leaves, const doubl
     const int per = (leaves + gridDim.x - 1) / gridDim.x;
     const int b0 = blockIdx.x * per;
     const int b1 = b0 + per < leaves ? b0 + per : leaves;
-    double acc[4] = {0, 0, 0, 0};
-#pragma unroll 4
-    for (int b = b0; b < b1; ++b) {
-        const double* G = side + (int64_t)b * 1024;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] += G[threadIdx.x + 256 * q];
-    }
+    double acc[4];
+    sum_blocks(side + (int64_t)b0 * 1024, b1 > b0 ? b1 - b0 : 0, threadIdx.x, acc);
 #pragma unroll
     for (int q = 0; q < 4; ++q) partial[(int64_t)blockIdx.x * 1024 + threadIdx.x + 256 * q] = acc[q];
 }
@@ -682,31 +702,6 @@ __device__ __forceinline__ void gram_chunk(const Raw4<T> (&c0)[S], const Raw4<T>
             }
         }
     }
-}
-
-// Sums `count` 32 x 32 FP64 blocks at src in index order (256 threads):
-// out[e] is entry t + 256 e of the sum.  Eight blocks are in flight per
-// step (the blocks come from other CUs, so every load goes to memory).
-__device__ __forceinline__ void sum_blocks(const double* __restrict__ src, int count, int t, double (&out)[4]) {
-    double s[4] = {0, 0, 0, 0};
-    int b = 0;
-#pragma unroll 1
-    for (; b + 8 <= count; b += 8) {
-        double v[8][4];
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) v[q][e] = src[(int64_t)(b + q) * 1024 + t + 256 * e];
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) s[e] += v[q][e];
-    }
-    for (; b < count; ++b)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) s[e] += src[(int64_t)b * 1024 + t + 256 * e];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) out[e] = s[e];
 }
 
 // One ticket per workgroup: returns (in every thread) whether this

@@ -104,8 +104,8 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d) {
 //   side     32 x 32 FP64 per superleaf (the Gram matrix, then R^{-1}), then
 //            three more: the whole matrix's R^{-1} on the flat path (T0),
 //            its R (T0+1) and the mixed path's residuals (T0+2)
-constexpr int kGramPartials = 128;
-constexpr int kGramPartials2 = 8;
+constexpr int kGramPartials = 1024;
+constexpr int kGramPartials2 = 32;
 constexpr int kCtlWords = 192;          // control words, including the stream tickets
 constexpr int kCtlStreamTickets = 64;   // ctl[64 ..]: tickets of the streaming Gram
 constexpr int kCtlStreamTickets2 = 128; // ctl[128 ..]: tickets of a second streaming pass
