@@ -58,7 +58,7 @@ extern "C" {
 #define TSQR_DOUBLE 2
 
 /* Level-0 leaf paths (tsqr_set_leaf_path). */
-#define TSQR_LEAF_AUTO 0        /* flat Gram path if the whole matrix passes the gate, else GRAM_TREE */
+#define TSQR_LEAF_AUTO 0        /* flat Gram path if the whole matrix passes the gate, else Householder */
 #define TSQR_LEAF_HOUSEHOLDER 1 /* Householder leaves and tree only */
 #define TSQR_LEAF_GRAM_TREE 2   /* Gram leaves (Householder where rejected), Householder tree */
 #define TSQR_LEAF_CHOLQR2 3     /* flat CholeskyQR + residual check + optional second pass, else tree */
@@ -183,10 +183,12 @@ int tsqr_batch_destroy(tsqr_batch *plan);
  *                          breaks down or ||R||_F ||R^{-1}||_F exceeds gate,
  *                          in which case that superleaf is Householder; the
  *                          tree above the superleaves is Householder;
- *   TSQR_LEAF_AUTO         as GRAM_TREE, except that when the sum of all
- *                          superleaf Grams itself passes the gate, R is its
+ *   TSQR_LEAF_AUTO         the Gram matrix of the whole matrix (FP64
+ *                          accumulation); if it passes the gate, R is its
  *                          Cholesky factor and Q = A R^{-1} (FP64), with no
- *                          tree (the same error bounds);
+ *                          tree (the same error bounds), else every
+ *                          superleaf is Householder (over a single
+ *                          superleaf: as GRAM_TREE);
  *   TSQR_LEAF_CHOLQR2      the whole matrix: R1 = chol(A^T A), Q1 = A R1^{-1}
  *                          with ||Q1^T Q1 - I||_F measured in the same pass;
  *                          above the tolerance a second CholeskyQR pass runs

@@ -384,21 +384,32 @@ __global__ __launch_bounds__(64) void tsqr_gram_chol(const TreeDesc d, T* __rest
 // the accumulator register v of (t, j-tile) is row 16v + 4(l/16) + t,
 // column j0 + l%16 of the output, so each lane stores 4 consecutive rows of
 // one column per (v, j-tile).  Products and sums in FP64, one rounding.
+// The grid is superleaves x parts: workgroup b takes part b % parts (a run
+// of whole 64-row blocks) of superleaf b / parts, so that few superleaves
+// still give many waves.  On the flat path every superleaf runs; otherwise
+// those flagged Householder are left to tsqr_super_q.
 template <typename T, bool VEC>
 __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restrict__ A, int lda,
                                                    const double* __restrict__ side, const T* __restrict__ blocks,
-                                                   const int* __restrict__ flags, const int* __restrict__ ctl) {
+                                                   const int* __restrict__ flags, const int* __restrict__ ctl,
+                                                   int parts) {
     // Highest superleaves first: the streaming R pass reads them last, so
     // part of them is still in the memory-side cache.
-    const int leaf = (int)gridDim.x - 1 - (int)blockIdx.x;
-    if (flags[leaf]) return;
+    const int item = (int)gridDim.x - 1 - (int)blockIdx.x;
+    const int leaf = item / parts;
+    const int part = item - leaf * parts;
+    const bool flat = ctl[0] != 0;
+    if (!flat && flags[leaf]) return;
     const int lane = threadIdx.x;
     const int n = d.n;
     const LevelDesc& L0 = d.lv[0];
     const int li = lane & 15, lq = lane >> 4;
+    const int leaf_rows = leaf_rows_of(L0, leaf);
+    const int per = (((leaf_rows + 63) >> 6) + parts - 1) / parts * 64;  // rows per part
+    const int first = part * per;
+    if (first >= leaf_rows) return;
     // ---- M = X S ----
     // flat path: X = R^{-1} of the whole matrix (side block T0) and S = I
-    const bool flat = ctl[0] != 0;
     const double* X = side + (int64_t)(flat ? L0.tiles : leaf) * 1024;  // row-major
     const bool root = flat || L0.tiles == 1;
     const T* S = nullptr;
@@ -430,8 +441,8 @@ __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restric
         }
     }
     // ---- Q rows = A rows * M ----
-    const int rows = leaf_rows_of(L0, leaf);
-    T* base = A + (int64_t)leaf * L0.tile_rows;
+    const int rows = leaf_rows - first < per ? leaf_rows - first : per;
+    T* base = A + (int64_t)leaf * L0.tile_rows + first;
     const int nk = (n + 3) / 4;  // k-groups holding real columns
     // raw[kg]: this lane's 4 input rows of column 4 kg + l/16 for the block
     // being computed; the next block's loads are issued before the MFMAs.

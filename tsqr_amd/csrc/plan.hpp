@@ -111,17 +111,23 @@ constexpr int kCtlStreamTickets = 64;   // ctl[64 ..]: tickets of the streaming 
 constexpr int kCtlStreamTickets2 = 128; // ctl[128 ..]: tickets of a second streaming pass
 
 // The streaming Gram kernel (tsqr_gram_stream, gram.hpp) runs P workgroups
-// of 4 waves, P = stream_groups(m): at most kStreamMaxGroups, and at least
-// four 64-row chunks per wave so that the loads of one chunk overlap the
-// MFMAs of the previous one.  Its partial sums take P blocks plus one per
-// group of kStreamGroup workgroups.
+// of 4 waves, P = stream_groups(m): at most kStreamMaxGroups, and four
+// 64-row chunks per wave, so that the loads of one chunk overlap the MFMAs
+// of the previous one, unless that leaves CUs idle (fewer than
+// kStreamFillGroups workgroups): then down to one chunk per wave.  Its
+// partial sums take P blocks plus one per group of kStreamGroup workgroups.
 constexpr int kStreamMaxGroups = 1024;
+constexpr int kStreamFillGroups = 256;
 constexpr int kStreamGroup = 32;
 static_assert(kStreamMaxGroups / kStreamGroup + 1 <= kCtlStreamTickets, "stream tickets do not fit");
 
 inline int stream_groups(int64_t m) {
     const int64_t chunks = (m + 63) / 64;
     int64_t p = (chunks + 15) / 16;
+    if (p < kStreamFillGroups) {
+        const int64_t one = (chunks + 3) / 4;  // one chunk per wave
+        p = one < kStreamFillGroups ? one : kStreamFillGroups;
+    }
     if (p < 1) p = 1;
     return (int)(p < kStreamMaxGroups ? p : kStreamMaxGroups);
 }

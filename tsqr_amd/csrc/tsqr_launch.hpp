@@ -60,7 +60,7 @@ inline bool plan_tsqr(int m, int n, TreeDesc* d) {
 
 // How the level-0 superleaves are factored.
 enum LeafPath {
-    kLeafAuto = 0,         // flat Gram path if the whole matrix passes the gate, else kLeafGramTree
+    kLeafAuto = 0,         // flat Gram path if the whole matrix passes the gate, else Householder
     kLeafHouseholder = 1,  // Householder for every superleaf
     kLeafGramTree = 2,     // Gram path per superleaf (Householder where rejected) + Householder tree
     kLeafCholQR2 = 3,      // flat CholeskyQR with a residual check and an optional second pass

@@ -35,10 +35,11 @@
 
 namespace tsqr {
 
-// Workgroups of the grid-stride fallback kernels (gram_g, gram_chol): enough
-// to fill the GPU when they run, few enough that a launch which finds the
-// flat path taken costs little.
+// Workgroups of the grid-stride Gram-tree kernels (gram_g, gram_chol):
+// enough to fill the GPU.
 constexpr int kFallbackGrid = 4096;
+// Waves the Q pass of the Gram paths aims for when superleaves are few.
+constexpr int kGramQWaves = 2048;
 
 namespace {
 // tickets = 0; flags = value; the Householder list is empty (value 0) or
@@ -99,8 +100,14 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     // pass, so the CholeskyQR2 setting runs as AUTO
     const int path = (o.r_only && sizeof(T) == 4 && o.path == kLeafCholQR2) ? kLeafAuto : o.path;
     const bool mixed = path == kLeafCholQR2;
-    const bool gram = path == kLeafAuto || path == kLeafGramTree;
-    const int force = gram ? 0 : 1;
+    // AUTO over two or more superleaves: the flat Gram path, with every
+    // superleaf listed Householder in case its gate fails.  GRAM_TREE (and
+    // AUTO over one superleaf): a Gram Cholesky per superleaf, Householder
+    // where that is rejected.
+    const bool flat_auto = path == kLeafAuto && leaves >= 2;
+    const bool gram_tree = path == kLeafGramTree || (path == kLeafAuto && leaves < 2);
+    const bool gram = flat_auto || gram_tree;
+    const int force = gram_tree ? 0 : 1;
     constexpr int NT = N > 16 ? 2 : 1;
 
     const int ninit = d.total_ctr > leaves ? d.total_ctr : leaves;
@@ -141,14 +148,13 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
             }
         }
     }
-    if (gram) {
-        if (leaves >= 2 && path == kLeafAuto) {
-            hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
-                               partial, stream_tk);
-            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
-                               (const double*)stream_sum, R, ldr, side, ctl, o.gate, (double*)nullptr);
-        }
-        // grid-stride kernels: on the flat path they only read ctl[0]
+    if (flat_auto) {
+        hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
+                           partial, stream_tk);
+        hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
+                           (const double*)stream_sum, R, ldr, side, ctl, o.gate, (double*)nullptr);
+    }
+    if (gram_tree) {
         const int gram_grid = leaves < kFallbackGrid ? leaves : kFallbackGrid;
         const int chol_grid = (leaves + 1) / 2 < kFallbackGrid ? (leaves + 1) / 2 : kFallbackGrid;
         hipLaunchKernelGGL((tsqr_gram_g<T, NT, VEC>), dim3(gram_grid), dim3(64), 0, st, d, (const T*)A, lda, side,
@@ -182,9 +188,15 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
                                o.tol);
         }
     }
-    if (gram)
-        hipLaunchKernelGGL((tsqr_gram_q<T, VEC>), dim3(leaves), dim3(64), 0, st, d, A, lda, (const double*)side,
-                           (const T*)blocks, (const int*)flags, (const int*)ctl);
+    if (gram) {
+        // few superleaves: split each into parts so that about
+        // kGramQWaves waves share the rows
+        const int tile_blocks = (d.lv[0].tile_rows + 63) / 64;
+        int parts = (kGramQWaves + leaves - 1) / leaves;
+        parts = parts < 1 ? 1 : (parts > tile_blocks ? tile_blocks : parts);
+        hipLaunchKernelGGL((tsqr_gram_q<T, VEC>), dim3(leaves * parts), dim3(64), 0, st, d, A, lda,
+                           (const double*)side, (const T*)blocks, (const int*)flags, (const int*)ctl, parts);
+    }
     hipLaunchKernelGGL((tsqr_super_q<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st, d, A,
                        lda, (const T*)blocks, (const int*)list, (const int*)ctl);
     if (timed) timing_mark(kPhaseLeafQ, false, st);
