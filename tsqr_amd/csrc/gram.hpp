@@ -577,50 +577,88 @@ __device__ __forceinline__ void sum_partials(GramLds& L, int nparts, const doubl
     }
 }
 
+// 1/sqrt(x) for x > 0 finite: the hardware estimate and two Newton steps
+// (y <- y + y (1/2 - (x/2) y^2)), within a few ulp of the rounded value.
+__device__ __forceinline__ double rsqrt_fp64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) y = __builtin_fma(y, __builtin_fma(-h * y, y, 0.5), y);
+    return y;
+}
+
+// Cholesky factor and its inverse in one sweep, by one wave.  Lane c < 32
+// starts from column c of the Gram matrix in L.G (identity outside the
+// leading n x n block) and ends with column c of R (zeros below the
+// diagonal); lane 32 + c starts from column c of the identity and, by the
+// same row operations (forward substitution with R^T), ends with column c
+// of R^{-T}, i.e. row c of R^{-1}.  Step j scales row j by 1/R_jj and
+// subtracts R_jr times it from every row r > j; R_jr is broadcast through
+// L.row.  The whole symmetric Schur complement is updated (no per-lane
+// select); its part below the diagonal is zeroed as each step reaches it.
+// Returns false (in every lane) if a pivot is not positive and finite; the
+// factor is then meaningless.
+__device__ __forceinline__ bool chol_inv_wave(GramLds& L, int n, int lane, double (&g)[32]) {
+    const int c = lane & 31;
+    const bool hi = lane >= 32;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) g[r] = (!hi && r < n && c < n) ? L.G[r][c] : (r == c ? 1.0 : 0.0);
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+        double piv = readlane_t<double>(g[j], j);
+        if (!(piv > 0.0) || !(piv < __builtin_inf())) {
+            ok = false;
+            piv = 1.0;
+        }
+        const double rd = rsqrt_fp64(piv);
+        g[j] = (hi || c >= j) ? g[j] * rd : 0.0;
+        if (!hi) L.row[c] = g[j];
+        wave_sync();
+#pragma unroll
+        for (int r = j + 1; r < 32; ++r) g[r] = fmat(-L.row[r], g[j], g[r]);
+        wave_sync();
+    }
+    return ok;
+}
+
 // Cholesky of the whole-matrix Gram matrix in L.G by one wave (lanes
-// 0..63; both half-waves factor the same matrix).  If it succeeds and
-// ||R||_F ||R^{-1}||_F <= gate: R (rounded to T) goes to the caller, R^{-1}
-// (FP64, row-major) to X, R itself in FP64 to rkeep (if not null) and
-// ctl[0] = 1.  Otherwise nothing is written.  Returns whether the gate
-// passed (the same in every lane).
+// 0..63).  If it succeeds and ||R||_F ||R^{-1}||_F <= gate: R (rounded to
+// T) goes to the caller, R^{-1} (FP64, row-major) to X, R itself in FP64
+// (row-major) to rkeep (if not null) and ctl[0] = 1.  Otherwise nothing is
+// written.  Returns whether the gate passed (the same in every lane).
 template <typename T>
 __device__ __forceinline__ bool flat_factor(GramLds& L, int n, T* __restrict__ R, int ldr, double* __restrict__ X,
                                             int* __restrict__ ctl, double gate, double* __restrict__ rkeep,
                                             int lane) {
     const int c = lane & 31;
-    double nr;
-    bool ok;
-    {
-        double g[32];
-        ok = chol_wave(L, n, lane, g);
-        nr = 0.0;
+    const bool hi = lane >= 32;
+    double g[32];
+    bool ok = chol_inv_wave(L, n, lane, g);
+    // ||R||_F^2 over lanes 0..n-1, ||R^{-1}||_F^2 over lanes 32..32+n-1
+    double s = 0.0;
 #pragma unroll
-        for (int r = 0; r < 32; ++r) nr = fmat(g[r], g[r], nr);
-        r_to_lds(g, L, lane);
-    }
-    double x[32];
-    tri_inv_wave(L, lane, x);
-    double nx = 0.0;
-#pragma unroll
-    for (int r = 0; r < 32; ++r) nx = fmat(x[r], x[r], nx);
-    if (c >= n) {
-        nr = 0.0;
-        nx = 0.0;
-    }
-    const double kest = __builtin_sqrt(sum_half(nr) * sum_half(nx));
+    for (int r = 0; r < 32; ++r) s = fmat(g[r], g[r], s);
+    if (c >= n) s = 0.0;
+    s = sum_half(s);
+    const double kest = __builtin_sqrt(readlane_t<double>(s, 0) * readlane_t<double>(s, 32));
     ok = ok && (kest <= gate);
-    if (!ok || lane >= 32) return ok;
-    if (c < n) {
-#pragma unroll 4
-        for (int r = 0; r < n; ++r) R[r + (int64_t)c * ldr] = (r <= c) ? (T)L.G[r][c] : T(0);
-    }
-    if (rkeep) {
-#pragma unroll 4
-        for (int r = 0; r < 32; ++r) rkeep[r * 32 + c] = L.G[r][c];
-    }
+    if (!ok) return ok;
+    if (!hi) {
+        if (c < n) {
 #pragma unroll
-    for (int r = 0; r < 32; ++r) X[r * 32 + c] = x[r];
-    if (c == 0) ctl[0] = 1;
+            for (int r = 0; r < 32; ++r)
+                if (r < n) R[r + (int64_t)c * ldr] = (r <= c) ? (T)g[r] : T(0);
+        }
+        if (rkeep) {
+#pragma unroll
+            for (int r = 0; r < 32; ++r) rkeep[r * 32 + c] = g[r];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) X[c * 32 + r] = g[r];
+        if (c == 0) ctl[0] = 1;
+    }
     return ok;
 }
 
