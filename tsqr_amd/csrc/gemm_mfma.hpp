@@ -425,13 +425,13 @@ __global__ __launch_bounds__(256) void tri_normal_solve(int n, int k, const T* _
     }
 }
 
-// Zeroes rows [r0, r0+nr) x columns [c0, c0+nc) of a column-major matrix.
+// Zeroes the entries below the diagonal of the n x n column-major M.
 template <typename T>
-__global__ void zero_block(T* __restrict__ M, int64_t ld, int r0, int nr, int c0, int nc) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nr * nc) return;
-    const int r = idx % nr, c = idx / nr;
-    M[(r0 + r) + (int64_t)(c0 + c) * ld] = T(0);
+__global__ void zero_lower(T* __restrict__ M, int64_t ld, int n) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * n) return;
+    const int r = (int)(idx % n), c = (int)(idx / n);
+    if (r > c) M[r + (int64_t)c * ld] = T(0);
 }
 
 // dst[r + c*ldd] += src[r + c*lds] for an nr x nc block.

@@ -770,6 +770,32 @@ __device__ __forceinline__ bool last_group(int* tk, int count, int* bcast) {
     return last;
 }
 
+// Tail of a streaming Gram launch (256 threads, P = gridDim.x workgroups),
+// after the workgroup's sum is in L.G (synchronised): L.G goes to
+// partial[blockIdx.x]; the last workgroup of each group of kStreamGroup
+// (ticket tk[g]) sums the group's partials in index order into
+// partial[P + g].
+__device__ __forceinline__ void stream_finish(GramLds& L, double* __restrict__ partial, int* __restrict__ tk,
+                                              int* bcast) {
+    const int t = threadIdx.x;
+    const int P = gridDim.x;
+    double* mine = partial + (int64_t)blockIdx.x * 1024;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int idx = t + 256 * e;
+        mine[idx] = L.G[idx >> 5][idx & 31];
+    }
+    const int g = blockIdx.x / kStreamGroup;
+    const int g0 = g * kStreamGroup;
+    const int members = P - g0 < kStreamGroup ? P - g0 : kStreamGroup;
+    if (!last_group(tk + g, members, bcast)) return;
+    double sum[4];
+    sum_blocks(partial + (int64_t)g0 * 1024, members, t, sum);
+    double* gsum = partial + (int64_t)(P + g) * 1024;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) gsum[t + 256 * e] = sum[e];
+}
+
 // End of a streaming Gram launch (256 threads, 4 waves, P = gridDim.x
 // workgroups): each wave holds its FP64 MFMA accumulators of the blocks
 // (0,0), (0,1), (1,1) of a 32 x 32 Gram matrix (lane l, register v: row
@@ -785,7 +811,6 @@ __device__ __forceinline__ void stream_reduce(GramLds& L, const v4d& a00, const 
     const int wv = t >> 6;
     const int col = lane & 15;
     const int i0 = lane >> 4;
-    const int P = gridDim.x;
 #pragma unroll 1
     for (int s = 0; s < 4; ++s) {
         if (wv == s) {
@@ -809,21 +834,7 @@ __device__ __forceinline__ void stream_reduce(GramLds& L, const v4d& a00, const 
         }
         __syncthreads();
     }
-    double* mine = partial + (int64_t)blockIdx.x * 1024;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int idx = t + 256 * e;
-        mine[idx] = L.G[idx >> 5][idx & 31];
-    }
-    const int g = blockIdx.x / kStreamGroup;
-    const int g0 = g * kStreamGroup;
-    const int members = P - g0 < kStreamGroup ? P - g0 : kStreamGroup;
-    if (!last_group(tk + g, members, bcast)) return;
-    double sum[4];
-    sum_blocks(partial + (int64_t)g0 * 1024, members, t, sum);
-    double* gsum = partial + (int64_t)(P + g) * 1024;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) gsum[t + 256 * e] = sum[e];
+    stream_finish(L, partial, tk, bcast);
 }
 
 // partial: P + ceil(P / kStreamGroup) blocks of 32 x 32 FP64; tk: the

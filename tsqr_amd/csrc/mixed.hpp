@@ -16,7 +16,8 @@
 // hundred rows at n = 32.
 //
 // Method (one flat CholeskyQR with an optional second pass, CholeskyQR2):
-//   pass 1  G1 = A^T A (BF16x3, above), summed over superleaves in FP64;
+//   pass 1  G1 = A^T A (BF16x3, above), streamed over the whole matrix
+//           (tsqr_mixed_stream), the 64-row partials summed in FP64;
 //           R1 = chol(G1) and kappa_est = ||R1||_F ||R1^{-1}||_F <= gate in
 //           FP64 (tsqr_gram_flat); Q1 = A R1^{-1} in FP64 MFMA, rounded once
 //           (tsqr_mixed_q), and in the same pass G2 = Q1^T Q1 of the rounded
@@ -107,8 +108,8 @@ __device__ __forceinline__ void load8(float (&x)[8], const float* __restrict__ b
     }
 }
 
-// Pass 1a: G_b = A_b^T A_b of every superleaf (one wave each) with BF16x3
-// products, written to side[leaf] (FP64, row-major 32 x 32).  Lane l loads
+// Pass 1a of the batch: G_b = A_b^T A_b of one superleaf (one wave) with
+// BF16x3 products, written to G (FP64, row-major 32 x 32).  Lane l loads
 // rows 8(l/16)..+7 of column l%16 (and 16 + l%16) of each 32-row step; the
 // two steps of a 64-row batch are loaded together and the next batch is in
 // flight while the current one is multiplied.
@@ -158,11 +159,94 @@ __device__ __forceinline__ void mixed_gram_body(const LevelDesc& L0, int n, cons
     g.store(G, lane);
 }
 
+// Pass 1a of a single matrix, streamed: G = A^T A with the BF16x3 products
+// of mixed_gram_body over the whole matrix by a persistent grid, as
+// tsqr_gram_stream does with FP64 products.  Workgroup b (4 waves) of P:
+// wave w = 4b + 0..3 takes the 64-row chunks w, w + 4P, ... (the last one
+// bounds-checked), each summed in FP32 on the matrix cores and added to the
+// wave's FP64 sums in LDS; the workgroup adds its waves in wave order into
+// partial[b] and the last of each group of kStreamGroup sums the group
+// (stream_finish).  Every sum has a fixed order.
 template <int NT, bool VEC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void tsqr_mixed_gram(const TreeDesc d, const float* __restrict__ A, int lda,
-                                                      double* __restrict__ side) {
-    __shared__ double sums[12 * 64];
-    mixed_gram_body<NT, VEC>(d.lv[0], d.n, A, lda, side + (int64_t)blockIdx.x * 1024, sums, blockIdx.x);
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void tsqr_mixed_stream(
+    const TreeDesc d, const float* __restrict__ A, int lda, double* __restrict__ partial, int* __restrict__ tk) {
+    __shared__ double sums[4][12 * 64];
+    __shared__ GramLds L;
+    __shared__ int bcast;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wv = t >> 6;
+    const int m = d.m, n = d.n;
+    const int W = 4 * (int)gridDim.x;
+    const int w = 4 * blockIdx.x + wv;
+    const int chunks = (m + 63) / 64;
+    const int col = lane & 15;
+    const int rq = 8 * (lane >> 4);
+    constexpr int D = 2;  // 32-row steps per chunk
+    GramAcc32 g;
+    g.sums = sums[wv];
+    g.zero(lane);
+    if (w < chunks) {
+        float c0[D][8], c1[D][8];
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            load8<VEC>(c0[q], A, lda, 64 * w + 32 * q + rq, col, m, n);
+            if constexpr (NT == 2) load8<VEC>(c1[q], A, lda, 64 * w + 32 * q + rq, col + 16, m, n);
+        }
+#pragma unroll 1
+        for (int ch = w;;) {
+            const int next = ch + W;
+            const bool more = next < chunks;
+            float p0[D][8], p1[D][8];
+            if (more) {
+#pragma unroll
+                for (int q = 0; q < D; ++q) {
+                    load8<VEC>(p0[q], A, lda, 64 * next + 32 * q + rq, col, m, n);
+                    if constexpr (NT == 2) load8<VEC>(p1[q], A, lda, 64 * next + 32 * q + rq, col + 16, m, n);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < D; ++q) g.add<NT>(c0[q], c1[q]);
+            g.flush<NT>(lane);
+            if (!more) break;
+            ch = next;
+#pragma unroll
+            for (int q = 0; q < D; ++q)
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    c0[q][k] = p0[q][k];
+                    if constexpr (NT == 2) c1[q][k] = p1[q][k];
+                }
+        }
+    }
+    // the workgroup's sum in L.G, waves added in order (accumulator layout:
+    // lane l, register v holds row 4(l/16) + v, column l%16 of a block)
+    const int i0 = 4 * (lane >> 4);
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+        if (wv == s) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + v;
+                const double g00 = sums[s][v * 64 + lane];
+                const double g01 = NT == 2 ? sums[s][(4 + v) * 64 + lane] : 0.0;
+                const double g11 = NT == 2 ? sums[s][(8 + v) * 64 + lane] : 0.0;
+                if (s == 0) {
+                    L.G[i][col] = g00;
+                    L.G[i][16 + col] = g01;
+                    L.G[16 + col][i] = g01;
+                    L.G[16 + i][16 + col] = g11;
+                } else {
+                    L.G[i][col] += g00;
+                    L.G[i][16 + col] += g01;
+                    L.G[16 + col][i] += g01;
+                    L.G[16 + i][16 + col] += g11;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    stream_finish(L, partial, tk, &bcast);
 }
 
 // Pass 1b / 2: Q = A X in place for every superleaf (X upper triangular,

@@ -297,7 +297,6 @@ int qr_rec(int m, int n, T* A, int lda, T* R, int ldr, char* ws, size_t panel_by
         hipLaunchKernelGGL(add_block<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, (const T*)tmp, n1, R12,
                            (int64_t)ldr, n1, n2);
     }
-    hipLaunchKernelGGL(zero_block<T>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, R, (int64_t)ldr, n1, n2, 0, n1);
     if (hipGetLastError() != hipSuccess) return TSQR_ERR_HIP;
     return qr_rec<T>(m, n2, A2, lda, R22, ldr, ws, panel_bytes, part, tmp, st, mixed);
 }
@@ -314,6 +313,10 @@ int qr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t wor
     const size_t panel = align256(tsqr_ws_bytes<T>(m, 32));
     const size_t parts = align256(tn_partial_elems(m, n) * sizeof(T));
     char* ws = reinterpret_cast<char*>(work);
+    // every R21 block of the recursion at once (the panels write their own
+    // diagonal blocks, zeros included)
+    hipLaunchKernelGGL(zero_lower<T>, dim3((unsigned)(((int64_t)n * n + 255) / 256)), dim3(256), 0, st, R,
+                       (int64_t)ldr, n);
     return qr_rec<T>(m, n, A, lda, R, ldr, ws, panel, reinterpret_cast<T*>(ws + panel),
                      reinterpret_cast<T*>(ws + panel + parts), st, mixed);
 }

@@ -125,12 +125,10 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     const double* X0 = side + (size_t)leaves * 1024;  // R^{-1} of the flat path
     if (mixed) {
         if constexpr (sizeof(T) == 4) {
-            hipLaunchKernelGGL((tsqr_mixed_gram<NT, VEC>), dim3(leaves), dim3(64), 0, st, d, (const float*)A, lda,
-                               side);
-            int np = 0;
-            const double* parts = reduce_grams(leaves, side, partial, nullptr, &np, st);
-            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, np, parts, R, ldr, side, ctl,
-                               o.gate, rkeep);
+            hipLaunchKernelGGL((tsqr_mixed_stream<NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const float*)A, lda,
+                               partial, stream_tk);
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
+                               (const double*)stream_sum, R, ldr, side, ctl, o.gate, rkeep);
         } else {
             hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
                                partial, stream_tk);
