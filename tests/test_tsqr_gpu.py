@@ -86,6 +86,23 @@ def test_shapes(m, n, dtype, path):
     check(A, Q, R, f"{m}x{n} {np.dtype(dtype).name} path={path} leaves={LAST_LEAF_STATS}", n_floor=floor)
 
 
+@pytest.mark.parametrize("path", [pytest.param(AUTO, id="auto"), pytest.param(CHOLQR2, id="cholqr2")])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("m,n", SHAPES)
+def test_flat_paths_forced_to_fall_back(m, n, dtype, path):
+    """The smallest gate (1; ||R||_F ||R^-1||_F >= sqrt(n) > 1 for n >= 2)
+    sends the flat paths to their Householder fallback, whose tree keeps each
+    node's explicit Q and whose superleaves form their S from the path to
+    the root (no down sweep): the same bounds as the Householder path, on
+    every shape (short last tiles, up to five tree levels)."""
+    A = O.gen_normal(m, n, dtype, seed=m * 5 + n)
+    with LeafPath(dtype, path, gate=1.0):
+        st, Q, R, _ = call("tsqr", A)
+    assert st == 0
+    assert n == 1 or not LAST_LEAF_STATS["flat"]
+    check(A, Q, R, f"{m}x{n} {np.dtype(dtype).name} path={path} forced fallback leaves={LAST_LEAF_STATS}")
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_padded_and_unaligned_lda(dtype):
     A = O.gen_uniform(9000, 32, dtype, seed=9)

@@ -101,10 +101,14 @@ __global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_r(const Tre
     }
 }
 
-template <typename T, int N, int RPL, bool VEC>
+// PATHS: the tree above holds each node's explicit Q (tsqr_tree_up with
+// FORMQ, no down sweep), and wave 0 forms this superleaf's S with path_s in
+// the LDS of waves 1 to 3, which are idle between the two barriers.
+template <typename T, int N, int RPL, bool VEC, bool PATHS = false>
 __device__ __forceinline__ void super_q_leaf(const TreeDesc& d, T* __restrict__ A, int lda,
                                              const T* __restrict__ blocks, SuperLds<T, N, RPL>& S, T* stash,
                                              int leaf) {
+    static_assert(SuperLds<T, N, RPL>::ST >= 4, "path_s borrows the LDS of waves 1 to 3");
     constexpr int H = 64 * RPL;
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -130,6 +134,10 @@ __device__ __forceinline__ void super_q_leaf(const TreeDesc& d, T* __restrict__ 
         form_q_tile<T, N, RPL>(y, S.wave[0], lane);
         if (L0.tiles == 1) {
             store_rows<T, N, RPL, true>(y, S.stack, H, srows, n, lane);
+        } else if constexpr (PATHS) {
+            const T* Sp = path_s<T, N>(d, blocks, leaf, S.wave[1].R, S.wave[2].R, S.wave[3].R, lane);
+            apply_s_store<T, N, RPL, true>(y, Sp, N, srows < n ? srows : n, n, S.stack, H, srows, S.wave[0].R,
+                                           lane);
         } else {
             const int parent = leaf / d.arity;
             const int slot = leaf - parent * d.arity;
@@ -145,7 +153,7 @@ __device__ __forceinline__ void super_q_leaf(const TreeDesc& d, T* __restrict__ 
                                       S.wave[w].R, lane);
 }
 
-template <typename T, int N, int RPL, bool VEC>
+template <typename T, int N, int RPL, bool VEC, bool PATHS = false>
 __global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_q(const TreeDesc d, T* __restrict__ A, int lda,
                                                                          const T* __restrict__ blocks,
                                                                          const int* __restrict__ list,
@@ -156,7 +164,7 @@ __global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_q(const Tre
     const int count = list[0];
 #pragma unroll 1
     for (int it = blockIdx.x; it < count; it += gridDim.x) {
-        super_q_leaf<T, N, RPL, VEC>(d, A, lda, blocks, S, stash, list[1 + it]);
+        super_q_leaf<T, N, RPL, VEC, PATHS>(d, A, lda, blocks, S, stash, list[1 + it]);
         __syncthreads();
     }
 }

@@ -163,9 +163,12 @@ __device__ __forceinline__ int tile_rows_of(const LevelDesc& L, int tile) {
 // of a parent to finish (ticket) goes on to factor the parent, and so on up
 // to the root.  Nothing waits: a wave that is not last simply exits.  The
 // root writes R to the caller and its explicit Q (the S factors of its
-// children) over its block.  tree_up_body does this for level-1 block
+// children) over its block.  With FORMQ every node writes its explicit Q
+// over its block instead of its reflectors, so that no down sweep is needed:
+// a superleaf's S is then the product of the blocks of its ancestors' Q that
+// lie on its path (path_s).  tree_up_body does this for level-1 block
 // `tile`; the kernel below maps one block per workgroup.
-template <typename T, int N, int RPL>
+template <typename T, int N, int RPL, bool FORMQ = false>
 __device__ __forceinline__ void tree_up_body(const TreeDesc& d, T* __restrict__ R, int ldr, T* __restrict__ blocks,
                                              int* __restrict__ tickets, TileLds<T, N>& L, int tile) {
     const int lane = threadIdx.x & 63;
@@ -180,12 +183,13 @@ __device__ __forceinline__ void tree_up_body(const TreeDesc& d, T* __restrict__ 
         factor_tile<T, N, RPL>(x, L, lane);
         const LevelDesc& Lv = d.lv[level];
         if (Lv.tiles == 1) break;
-        store_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
         const LevelDesc& P = d.lv[level + 1];
         const int parent = tile / d.arity;
         const int slot = tile - parent * d.arity;
         T* pblk = blocks + P.blk_off + (int64_t)parent * d.H * n;
         store_r<T, N>(L, pblk + slot * n, d.H, rows < n ? rows : n, n, lane);
+        if constexpr (FORMQ) form_q_tile<T, N, RPL>(x, L, lane);
+        store_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         int old = 0;
@@ -209,13 +213,70 @@ __device__ __forceinline__ void tree_up_body(const TreeDesc& d, T* __restrict__ 
     store_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
 }
 
-template <typename T, int N, int RPL>
+template <typename T, int N, int RPL, bool FORMQ = false>
 __global__ __launch_bounds__(64) void tsqr_tree_up(const TreeDesc d, T* __restrict__ R, int ldr,
                                                     T* __restrict__ blocks, int* __restrict__ tickets,
                                                     const int* __restrict__ skip) {
     __shared__ TileLds<T, N> L;
     if (skip[0]) return;
-    tree_up_body<T, N, RPL>(d, R, ldr, blocks, tickets, L, blockIdx.x);
+    tree_up_body<T, N, RPL, FORMQ>(d, R, ldr, blocks, tickets, L, blockIdx.x);
+}
+
+// S of superleaf `leaf` after a FORMQ up sweep: B_1 B_2 ... B_top, B_l the
+// n x n block (ld H) of the level-l ancestor's explicit Q at the rows of the
+// level-(l-1) ancestor, accumulated from the root down (X <- B_l X).  One
+// wave; X, Y and B are N x N LDS buffers (column-major, ld N), entries
+// outside n x n zero.  Returns the buffer (X or Y) that holds the result;
+// as with the down sweep, its rows beyond min(rows, n) of a short superleaf
+// are not meaningful and the caller uses only the first min(rows, n).
+template <typename T, int N>
+__device__ __forceinline__ T* path_s(const TreeDesc& d, const T* __restrict__ blocks, int leaf, T* X, T* Y, T* B,
+                                     int lane) {
+    constexpr int PER = (N * N + 63) / 64;
+    const int n = d.n;
+    const int top = d.levels - 1;
+    int64_t pw = 1;  // arity^(l - 1) for the level l being applied
+    for (int l = 1; l < top; ++l) pw *= d.arity;
+#pragma unroll 1
+    for (int l = top; l >= 1; --l) {
+        const int64_t child = leaf / pw;  // ancestor at level l - 1
+        const int64_t anc = child / d.arity;
+        const int slot = (int)(child - anc * d.arity);
+        const T* blk = blocks + d.lv[l].blk_off + anc * d.H * n + (int64_t)slot * n;
+        T* dst = l == top ? X : B;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int idx = lane + 64 * q;
+            const int k = idx % N, c = idx / N;
+            if (idx < N * N) dst[idx] = (k < n && c < n) ? blk[k + (int64_t)c * d.H] : T(0);
+        }
+        wave_sync();
+        if (l < top) {
+            // only the first min(rows, n) columns of the ancestor's Q (and
+            // rows of X) exist; the block rows beyond a short child's R are
+            // never written, so they are left out of the sum
+            const int arows = tile_rows_of(d.lv[l], (int)anc);
+            const int kmax = arows < n ? arows : n;
+#pragma unroll 1
+            for (int q = 0; q < PER; ++q) {
+                const int idx = lane + 64 * q;
+                const int i = idx % N, c = idx / N;
+                T acc = T(0);
+                if (idx < N * N) {
+#pragma unroll 8
+                    for (int k = 0; k < N; ++k)
+                        if (k < kmax) acc = fmat(B[i + k * N], X[k + c * N], acc);
+                    Y[idx] = acc;
+                }
+            }
+            wave_sync();
+            T* t = X;
+            X = Y;
+            Y = t;
+        }
+        pw /= d.arity;
+    }
+    return X;
 }
 
 // A single tile (m <= H): factor, R to the caller, Q over A (unless form_q

@@ -163,17 +163,27 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     hipLaunchKernelGGL((tsqr_super_r<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st, d,
                        (const T*)A, lda, R, ldr, blocks, (const int*)list, (const int*)ctl);
     if (timed) timing_mark(kPhaseLeafR, false, st);
+    // Where the Householder kernels only back up a flat path, the tree keeps
+    // every node's explicit Q and each superleaf forms its own S (path_s), so
+    // no down-sweep launches (one per level) are queued for a path that
+    // nearly never runs.
+    const bool flat_first = flat_auto || mixed;
     if (d.levels >= 2) {
         if (timed) timing_mark(kPhaseTree, true, st);
-        hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL>), dim3(d.lv[1].tiles), dim3(64), 0, st, d, R, ldr, blocks,
-                           tickets, (const int*)ctl);
+        if (flat_first)
+            hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, true>), dim3(d.lv[1].tiles), dim3(64), 0, st, d, R, ldr,
+                               blocks, tickets, (const int*)ctl);
+        else
+            hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, false>), dim3(d.lv[1].tiles), dim3(64), 0, st, d, R, ldr,
+                               blocks, tickets, (const int*)ctl);
         if (timed) timing_mark(kPhaseTree, false, st);
     }
     if (o.r_only) return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
     if (timed) timing_mark(kPhaseDown, true, st);
-    for (int l = d.levels - 2; l >= 1; --l)
-        hipLaunchKernelGGL((tsqr_down<T, N, RPL>), dim3(d.lv[l].tiles), dim3(64), 0, st, d, l, blocks,
-                           (const int*)ctl);
+    if (!flat_first)
+        for (int l = d.levels - 2; l >= 1; --l)
+            hipLaunchKernelGGL((tsqr_down<T, N, RPL>), dim3(d.lv[l].tiles), dim3(64), 0, st, d, l, blocks,
+                               (const int*)ctl);
     if (timed) timing_mark(kPhaseDown, false, st);
     if (timed) timing_mark(kPhaseLeafQ, true, st);
     if (mixed) {
@@ -195,8 +205,12 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
         hipLaunchKernelGGL((tsqr_gram_q<T, VEC>), dim3(leaves * parts), dim3(64), 0, st, d, A, lda,
                            (const double*)side, (const T*)blocks, (const int*)flags, (const int*)ctl, parts);
     }
-    hipLaunchKernelGGL((tsqr_super_q<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st, d, A,
-                       lda, (const T*)blocks, (const int*)list, (const int*)ctl);
+    if (flat_first)
+        hipLaunchKernelGGL((tsqr_super_q<T, N, RPL, VEC, true>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0,
+                           st, d, A, lda, (const T*)blocks, (const int*)list, (const int*)ctl);
+    else
+        hipLaunchKernelGGL((tsqr_super_q<T, N, RPL, VEC, false>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0,
+                           st, d, A, lda, (const T*)blocks, (const int*)list, (const int*)ctl);
     if (timed) timing_mark(kPhaseLeafQ, false, st);
     return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
 }
