@@ -15,8 +15,9 @@ metric  TSQR HBM GB/s = B_alg / (seconds per step), with
         lower bound for an explicit Q that does not fit on chip).
 roofline  for the dominant phase of the two streaming passes (leaf R pass:
         reads A, m*n*4 algorithmic bytes; leaf Q pass: reads A and writes Q,
-        2*m*n*4), from HIP events on the launch stream over the timed steps,
-        against the 8 TB/s HBM peak.
+        2*m*n*4), from HIP events on the launch stream over a second pass of
+        the same K steps (the marks are kept out of the timed pass: each costs
+        a few microseconds of GPU time), against the 8 TB/s HBM peak.
 cpu_baseline  the oracle's C restatement of the reference algorithm
         (OpenMP over tiles, all host cores) on a bounded sample of the same
         workload (m/8 rows), plus OpenBLAS sgeqrf+sorgqr on the same sample.
@@ -352,13 +353,22 @@ def main():
         step()
     torch.cuda.synchronize()
 
-    L.tsqr_timing_enable(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # Per-kernel durations: the same K steps again with HIP event marks that
+    # the library records on its launch stream around each phase.  They are
+    # kept out of the pass timed above because a mark costs a few
+    # microseconds of GPU time, which is noticeable next to the small
+    # workloads' kernels.
+    L.tsqr_timing_enable(1)
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
     phases = timing_read()
     L.tsqr_timing_enable(0)
     print(f"# phases (ms total, count): {phases}", file=sys.stderr, flush=True)

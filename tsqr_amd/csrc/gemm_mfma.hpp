@@ -131,20 +131,38 @@ __global__ __launch_bounds__(64) void gemm_tn_mfma(int m, int n1, int n2, const 
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = typename M::v4{0, 0, 0, 0};
+    // the next 16-row step's loads are issued before this step's MFMAs (a
+    // launch has few waves per SIMD, so each wave hides its own latency)
+    Four<T> q[4], x[4];
+    if (kbeg < kend) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) q[a] = load_four<T, VEC>(Q, ldq, kbeg + 4 * lq, i0 + 16 * a + li, kend, n1);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) x[b] = load_four<T, VEC>(A, lda, kbeg + 4 * lq, j0 + 16 * b + li, kend, n2);
+    }
 #pragma unroll 1
     for (int64_t r0 = kbeg; r0 < kend; r0 += 16) {
-        const int64_t r = r0 + 4 * lq;
-        Four<T> q[4], x[4];
+        const int64_t rn = r0 + 16 + 4 * lq;
+        const bool more = r0 + 16 < kend;
+        Four<T> qn[4], xn[4];
+        if (more) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a) q[a] = load_four<T, VEC>(Q, ldq, r, i0 + 16 * a + li, kend, n1);
+            for (int a = 0; a < 4; ++a) qn[a] = load_four<T, VEC>(Q, ldq, rn, i0 + 16 * a + li, kend, n1);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) x[b] = load_four<T, VEC>(A, lda, r, j0 + 16 * b + li, kend, n2);
+            for (int b = 0; b < 4; ++b) xn[b] = load_four<T, VEC>(A, lda, rn, j0 + 16 * b + li, kend, n2);
+        }
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b) acc[a][b] = M::mma(q[a].v[t], x[b].v[t], acc[a][b]);
+        if (!more) break;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            q[a] = qn[a];
+            x[a] = xn[a];
+        }
     }
     T* out = part + (int64_t)blockIdx.x * n1 * n2;
 #pragma unroll
@@ -162,7 +180,8 @@ __global__ __launch_bounds__(64) void gemm_tn_mfma(int m, int n1, int n2, const 
 // part[k][i + j*n1].  256 threads per 16 outputs: thread (o, g) adds the
 // chunks k = g, g + 16, ... of output o in order, then the 16 group sums
 // are added in order of g, so the result is deterministic and no thread
-// walks more than chunks / 16 partials.
+// walks more than chunks / 16 partials.  Eight partials are loaded before
+// they are added (the same order), so that the loads overlap.
 template <typename T>
 __global__ __launch_bounds__(256) void gemm_tn_reduce(int n1, int n2, int chunks, const T* __restrict__ part,
                                                       T* __restrict__ C, int64_t ldc, int accumulate) {
@@ -171,8 +190,18 @@ __global__ __launch_bounds__(256) void gemm_tn_reduce(int n1, int n2, int chunks
     const int idx = blockIdx.x * 16 + o;
     const int total = n1 * n2;
     T s = T(0);
-    if (idx < total)
-        for (int k = g; k < chunks; k += 16) s += part[(int64_t)k * total + idx];
+    if (idx < total) {
+        int k = g;
+#pragma unroll 1
+        for (; k + 7 * 16 < chunks; k += 8 * 16) {
+            T v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = part[(int64_t)(k + 16 * q) * total + idx];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s += v[q];
+        }
+        for (; k < chunks; k += 16) s += part[(int64_t)k * total + idx];
+    }
     red[o][g] = s;
     __syncthreads();
     if (g != 0 || idx >= total) return;
@@ -276,21 +305,30 @@ __global__ __launch_bounds__(64) void gemm_tn_bf16x3(int m, int n1, int n2, cons
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f4v{0, 0, 0, 0};
+    // the next 32-row step's loads are issued before this step's splits and
+    // MFMAs (few waves per SIMD: each wave hides its own latency)
+    float vq[4][8], vx[4][8];
+    if (kbeg < kend) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) load_eight<VEC>(vq[a], Q, ldq, kbeg + 8 * lq, i0 + 16 * a + li, kend, n1);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) load_eight<VEC>(vx[b], A, lda, kbeg + 8 * lq, j0 + 16 * b + li, kend, n2);
+    }
 #pragma unroll 1
     for (int64_t r0 = kbeg; r0 < kend; r0 += 32) {
-        const int64_t r = r0 + 8 * lq;
+        const int64_t rn = r0 + 32 + 8 * lq;
+        const bool more = r0 + 32 < kend;
         Split3 sq[4], sx[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            float v[8];
-            load_eight<VEC>(v, Q, ldq, r, i0 + 16 * a + li, kend, n1);
-            sq[a] = split3(v);
+            sq[a] = split3(vq[a]);
+            sx[a] = split3(vx[a]);
         }
+        if (more) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            float v[8];
-            load_eight<VEC>(v, A, lda, r, j0 + 16 * b + li, kend, n2);
-            sx[b] = split3(v);
+            for (int a = 0; a < 4; ++a) load_eight<VEC>(vq[a], Q, ldq, rn, i0 + 16 * a + li, kend, n1);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) load_eight<VEC>(vx[b], A, lda, rn, j0 + 16 * b + li, kend, n2);
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a)

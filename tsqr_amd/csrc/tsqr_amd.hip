@@ -179,12 +179,16 @@ int tsqr_r_ref(int m, int n, const T* A, int lda, T* R, int ldr, T* work, int ld
 inline int split_cols(int n) { return 32 * ((n + 63) / 64); }
 
 // Rows per split-K chunk of R12 = Q1^T A2: enough chunks that the launch has
-// about g_tn_waves waves (512 unless tuned), each chunk a multiple of 16
-// rows and at least 64 rows.
+// about W waves, each chunk a multiple of 16 rows and at least 64 rows.  W
+// is g_tn_waves (512 unless tuned) for short panels and grows with m, about
+// one wave per 512 rows up to 4 g_tn_waves (measured best at m = 27,648 and
+// at m = 1,048,576; a launch has at most one wave per SIMD either way).
 inline int tn_chunk_rows(int m, int n1, int n2) {
     const int tiles = ((n1 + 63) / 64) * ((n2 + 63) / 64);
-    const int waves = g_tn_waves.load(std::memory_order_relaxed);
-    const int chunks = (waves + tiles - 1) / tiles;
+    const int64_t base = g_tn_waves.load(std::memory_order_relaxed);
+    int64_t waves = (int64_t)m / 512;
+    waves = waves < base ? base : (waves > 4 * base ? 4 * base : waves);
+    const int chunks = (int)((waves + tiles - 1) / tiles);
     int kc = (m + chunks - 1) / chunks;
     kc = (kc + 15) / 16 * 16;
     return kc < 64 ? 64 : kc;
