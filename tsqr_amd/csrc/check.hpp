@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void tsqr_resid_stream(int m, int n, const T* 
 }
 
 // Enqueues the three Gram reductions on st.  On completion the group sums
-// of G_A, G_Q and G_E (ngroups = ceil(P / kStreamGroup) blocks of 32 x 32
+// of G_A, G_Q and G_E (ngroups = stream_group_count(P) blocks of 32 x 32
 // FP64 each, P = stream_groups(m)) are at the returned offsets of work; the
 // caller adds each set in order.  n <= 32.
 template <typename T, int NT, bool VEC>

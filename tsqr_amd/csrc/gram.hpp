@@ -688,12 +688,12 @@ __global__ __launch_bounds__(kSumThreads) void tsqr_gram_flat(const TreeDesc d, 
 // rows of FP32, 32 of FP64) w, w + 4P, w + 8P, ... in FP64 MFMA registers (the interleaving keeps
 // every wave on nearby addresses and all of them busy to the end).  The
 // workgroup then sums its 4 waves in LDS (in wave order) into partial[b].
-// Workgroups form groups of kStreamGroup; the last of a group to finish
+// Workgroups form groups (stream_group_size); the last of a group to finish
 // (ticket; no wave ever waits on another) sums the group's partials in index
-// order into partial[P + g].  tsqr_gram_flat then sums the ceil(P /
-// kStreamGroup) group partials in order and factors the result.  Every sum
-// runs in a fixed order, so the result does not depend on which workgroup
-// arrives last.  P depends only on m (stream_groups, plan.hpp).
+// order into partial[P + g].  tsqr_gram_flat then sums the
+// stream_group_count(P) group partials in order and factors the result.
+// Every sum runs in a fixed order, so the result does not depend on which
+// workgroup arrives last.  P depends only on m (stream_groups, plan.hpp).
 
 // Slabs of 16 rows per streamed chunk: 64 rows of FP32 or 32 rows of FP64,
 // so that a chunk and its prefetched successor take 64 VGPRs either way.
@@ -772,9 +772,9 @@ __device__ __forceinline__ bool last_group(int* tk, int count, int* bcast) {
 
 // Tail of a streaming Gram launch (256 threads, P = gridDim.x workgroups),
 // after the workgroup's sum is in L.G (synchronised): L.G goes to
-// partial[blockIdx.x]; the last workgroup of each group of kStreamGroup
-// (ticket tk[g]) sums the group's partials in index order into
-// partial[P + g].
+// partial[blockIdx.x]; the last workgroup of each group of G =
+// stream_group_size(P) (ticket tk[g]) sums the group's partials in index
+// order into partial[P + g].
 __device__ __forceinline__ void stream_finish(GramLds& L, double* __restrict__ partial, int* __restrict__ tk,
                                               int* bcast) {
     const int t = threadIdx.x;
@@ -785,9 +785,10 @@ __device__ __forceinline__ void stream_finish(GramLds& L, double* __restrict__ p
         const int idx = t + 256 * e;
         mine[idx] = L.G[idx >> 5][idx & 31];
     }
-    const int g = blockIdx.x / kStreamGroup;
-    const int g0 = g * kStreamGroup;
-    const int members = P - g0 < kStreamGroup ? P - g0 : kStreamGroup;
+    const int G = stream_group_size(P);
+    const int g = blockIdx.x / G;
+    const int g0 = g * G;
+    const int members = P - g0 < G ? P - g0 : G;
     if (!last_group(tk + g, members, bcast)) return;
     double sum[4];
     sum_blocks(partial + (int64_t)g0 * 1024, members, t, sum);
@@ -800,9 +801,7 @@ __device__ __forceinline__ void stream_finish(GramLds& L, double* __restrict__ p
 // workgroups): each wave holds its FP64 MFMA accumulators of the blocks
 // (0,0), (0,1), (1,1) of a 32 x 32 Gram matrix (lane l, register v: row
 // 4v + l/16, column l%16 of the block).  The workgroup sums its waves in
-// wave order into partial[blockIdx.x]; the last workgroup of each group of
-// kStreamGroup (ticket tk[g]) sums the group's partials in index order into
-// partial[P + g].
+// wave order into partial[blockIdx.x]; then stream_finish.
 template <int NT>
 __device__ __forceinline__ void stream_reduce(GramLds& L, const v4d& a00, const v4d& a01, const v4d& a11,
                                               double* __restrict__ partial, int* __restrict__ tk, int* bcast) {
@@ -837,7 +836,7 @@ __device__ __forceinline__ void stream_reduce(GramLds& L, const v4d& a00, const 
     stream_finish(L, partial, tk, bcast);
 }
 
-// partial: P + ceil(P / kStreamGroup) blocks of 32 x 32 FP64; tk: the
+// partial: P + stream_group_count(P) blocks of 32 x 32 FP64; tk: the
 // kCtlStreamTickets ticket words, zero on entry.
 template <typename T, int NT, bool VEC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void tsqr_gram_stream(const TreeDesc d, const T* __restrict__ A, int lda,

@@ -165,7 +165,7 @@ __device__ __forceinline__ void mixed_gram_body(const LevelDesc& L0, int n, cons
 // wave w = 4b + 0..3 takes the 64-row chunks w, w + 4P, ... (the last one
 // bounds-checked), each summed in FP32 on the matrix cores and added to the
 // wave's FP64 sums in LDS; the workgroup adds its waves in wave order into
-// partial[b] and the last of each group of kStreamGroup sums the group
+// partial[b] and the last of each group (stream_group_size) sums the group
 // (stream_finish).  Every sum has a fixed order.
 template <int NT, bool VEC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void tsqr_mixed_stream(

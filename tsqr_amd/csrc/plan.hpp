@@ -115,11 +115,20 @@ constexpr int kCtlStreamTickets2 = 128; // ctl[128 ..]: tickets of a second stre
 // 64-row chunks per wave, so that the loads of one chunk overlap the MFMAs
 // of the previous one, unless that leaves CUs idle (fewer than
 // kStreamFillGroups workgroups): then down to one chunk per wave.  Its
-// partial sums take P blocks plus one per group of kStreamGroup workgroups.
+// partial sums take P blocks plus one per group (stream_group_size).
 constexpr int kStreamMaxGroups = 1024;
 constexpr int kStreamFillGroups = 256;
-constexpr int kStreamGroup = 32;
+constexpr int kStreamGroup = 32;  // the largest group
 static_assert(kStreamMaxGroups / kStreamGroup + 1 <= kCtlStreamTickets, "stream tickets do not fit");
+
+// Workgroups per group of the streaming Gram's partial sums for a grid of P:
+// small grids use small groups, so that the last workgroup of a group and
+// the final sum each add only a few partials (at most 32 groups either way).
+constexpr int stream_group_size(int P) { return P <= 64 ? 8 : (P <= 256 ? 16 : kStreamGroup); }
+constexpr int stream_group_count(int P) { return (P + stream_group_size(P) - 1) / stream_group_size(P); }
+static_assert(stream_group_count(64) <= 32 && stream_group_count(256) <= 32 &&
+                  stream_group_count(kStreamMaxGroups) <= 32,
+              "at most 32 stream groups");
 
 inline int stream_groups(int64_t m) {
     const int64_t chunks = (m + 63) / 64;
@@ -134,7 +143,7 @@ inline int stream_groups(int64_t m) {
 
 inline size_t stream_partials(int64_t m) {
     const int p = stream_groups(m);
-    return (size_t)(p + (p + kStreamGroup - 1) / kStreamGroup);
+    return (size_t)(p + stream_group_count(p));
 }
 
 // Workspace of the device-side checker (check.hpp): three partial areas of

@@ -369,7 +369,7 @@ int check_impl(int m, int n, const T* A, int lda, const T* Q, int ldq, const T* 
     size_t off[3];
     e = check_launch<T>(m, n, A, lda, Q, ldq, R, ldr, work, off, st);
     if (e) return e;
-    const int groups = (stream_groups(m) + kStreamGroup - 1) / kStreamGroup;
+    const int groups = stream_group_count(stream_groups(m));
     std::vector<double> host((size_t)3 * groups * 1024);
     for (int k = 0; k < 3; ++k)
         if (hipMemcpyAsync(host.data() + (size_t)k * groups * 1024, reinterpret_cast<const char*>(work) + off[k],

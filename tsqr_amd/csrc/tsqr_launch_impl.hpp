@@ -120,7 +120,7 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     // streaming Gram: sgroups workgroups, then sgroup_parts group partials
     int* stream_tk = ctl + kCtlStreamTickets;
     const int sgroups = stream_groups(d.m);
-    const int sgroup_parts = (sgroups + kStreamGroup - 1) / kStreamGroup;
+    const int sgroup_parts = stream_group_count(sgroups);
     double* stream_sum = partial + (size_t)sgroups * 1024;
     const double* X0 = side + (size_t)leaves * 1024;  // R^{-1} of the flat path
     if (mixed) {
