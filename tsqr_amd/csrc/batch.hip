@@ -44,7 +44,7 @@ bool make_prob(int m, int n, void* A, int lda, void* R, int ldr, int method, siz
     p->N = N;
     p->vec = ((uintptr_t)A % 16 == 0) && (((size_t)lda * sizeof(T)) % 16 == 0);
     p->mixed = method == TSQR_BATCH_MIXED;
-    p->pad = 0;
+    p->pre_gram = 0;
     return true;
 }
 
@@ -87,6 +87,17 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
             couples.push_back(c);
         }
     }
+    // Mixed FP32: where the second panel's superleaves are the couple's row
+    // blocks, batch_project forms their Gram matrices as it writes the
+    // projected rows, and the second stage's Gram launch skips that panel.
+    for (size_t k = 0; k < couples.size(); ++k) {
+        BatchCouple& c = couples[k];
+        BatchProb& q = second[k];
+        c.gram = (sizeof(T) == 4 && q.mixed && q.d.lv[0].tile_rows == c.leaf_rows && q.d.lv[0].tiles == c.leaves)
+                     ? 1
+                     : 0;
+        q.pre_gram = c.gram;
+    }
     std::vector<BatchProb> all(first);
     all.insert(all.end(), second.begin(), second.end());
     size_t leaves_in[2] = {0, 0};
@@ -94,7 +105,12 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
     const size_t list_bytes[2] = {align256(8 + leaves_in[0] * sizeof(int2)), align256(8 + leaves_in[1] * sizeof(int2))};
     const size_t head = list_bytes[0] + list_bytes[1];
     for (auto& p : all) p.ws += (int64_t)head;
-    for (auto& c : couples) c.ws += (int64_t)head;
+    for (size_t k = 0; k < couples.size(); ++k) {
+        BatchCouple& c = couples[k];
+        c.ws += (int64_t)head;
+        const BatchProb& q = all[first.size() + k];
+        c.side2 = q.ws + (int64_t)q.w.side;
+    }
     off += head;
 
     // item lists (problem indices into `all`)

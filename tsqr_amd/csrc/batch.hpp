@@ -56,7 +56,7 @@ struct BatchProb {
     int lda, ldr;
     int N, vec;    // padded columns; A and lda allow 16-byte loads
     int mixed;     // 1: mixed method (FP32), 0: Householder
-    int pad;
+    int pre_gram;  // 1: batch_project already wrote the superleaf Grams
 };
 
 // Coupling of the two panels of one matrix with 32 < n <= 64.
@@ -67,7 +67,11 @@ struct BatchCouple {
     int64_t ws;      // byte offset: leaves + 2 FP64 32 x 32 blocks, then 4 ints
     int m, n2, lda, ldr;
     int leaves, leaf_rows;
-    int vec, pad;
+    int vec;
+    int gram;       // 1: batch_project also writes the BF16x3 Gram matrix of
+                    // each new A2 row block, the second panel's superleaf b,
+                    // to the second panel's side block b (at byte side2)
+    int64_t side2;
 };
 
 // Per-couple workspace: block b < leaves is row block b's product, block
@@ -324,6 +328,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ba
     __shared__ double sums[12 * 64];
     const int2 it = items[blockIdx.x];
     const BatchProb& p = probs[it.x];
+    if (p.pre_gram) return;  // written by batch_project
     double* side = prob_side(p, work);
     const LevelDesc& L0 = p.d.lv[0];
     const int n = p.d.n;
@@ -579,8 +584,18 @@ __global__ __launch_bounds__(256) void batch_cross_sum(const BatchCouple* __rest
 // * 4 + t (16(l/16) + 4v + t for float, 16v + 4(l/16) + t for double),
 // column 16 jt + l%16, and each lane stores 4 consecutive rows per (v, jt).
 // The next block's loads are issued before the current block's MFMAs.
-template <typename T, bool VEC>
-__device__ __forceinline__ void project_block(const BatchCouple& q, int blk, const double* __restrict__ P, int lane) {
+//
+// With GRAM (float, q.gram set) the new rows also go through the BF16x3
+// Gram products of the mixed method (GramAcc32, FP32 per 64 rows, then FP64
+// sums in `sums`) straight from the accumulators: lane (li, lq) holds rows
+// 16 lq .. 16 lq + 15 of columns li and 16 + li of each 64-row block and
+// feeds rows 16 lq + 8 h + s to k-slot s of the h-th 32-row product, the
+// same assignment in every lane.  The result is the second panel's Gram
+// matrix of superleaf blk, written to G.
+template <typename T, bool VEC, bool GRAM = false>
+__device__ __forceinline__ void project_block(const BatchCouple& q, int blk, const double* __restrict__ P, int lane,
+                                              double* sums = nullptr, double* __restrict__ G = nullptr) {
+    static_assert(!GRAM || sizeof(T) == 4, "the fused Gram is the mixed (FP32) method's");
     using M = Mfma16<T>;
     const int li = lane & 15, lq = lane >> 4;
     const int64_t r0 = (int64_t)blk * q.leaf_rows;
@@ -604,6 +619,11 @@ __device__ __forceinline__ void project_block(const BatchCouple& q, int blk, con
             for (int v = 0; v < 4; ++v)
                 cv[jt][v] = load_four<T, VEC>(Ab, q.lda, s0 + row_of(v), 16 * jt + li, rows, q.n2);
     };
+    GramAcc32 g;
+    if constexpr (GRAM) {
+        g.sums = sums;
+        g.zero(lane);
+    }
     if (rows > 0) load_block(0);
 #pragma unroll 1
     for (int s0 = 0; s0 < rows; s0 += 64) {
@@ -634,7 +654,22 @@ __device__ __forceinline__ void project_block(const BatchCouple& q, int blk, con
                 store_four<T, VEC>(o, Ab, q.lda, s0 + row_of(v), 16 * jt + li, rows, q.n2);
             }
         }
+        if constexpr (GRAM) {
+            // rows past the end and columns past n2 are zero in acc
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                float x0[8], x1[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    x0[e] = acc[e & 3][0][2 * h + (e >> 2)];
+                    x1[e] = acc[e & 3][1][2 * h + (e >> 2)];
+                }
+                g.template add<2>(x0, x1);
+            }
+            g.template flush<2>(lane);
+        }
     }
+    if constexpr (GRAM) g.store(G, lane);
 }
 
 template <typename T>
@@ -645,6 +680,17 @@ __global__ __launch_bounds__(64, 2) void batch_project(const BatchCouple* __rest
     const BatchCouple& q = couples[it.x];
     if (pass == 2 && !reinterpret_cast<const int*>(work + q.ws + couple_ints_off(q.leaves))[kCoupleReorth]) return;
     const double* P = reinterpret_cast<const double*>(work + q.ws) + (int64_t)q.leaves * 1024;
+    if constexpr (sizeof(T) == 4) {
+        if (q.gram) {
+            __shared__ double sums[12 * 64];
+            double* G = reinterpret_cast<double*>(work + q.side2) + (int64_t)it.y * 1024;
+            if (q.vec)
+                project_block<T, true, true>(q, it.y, P, threadIdx.x, sums, G);
+            else
+                project_block<T, false, true>(q, it.y, P, threadIdx.x, sums, G);
+            return;
+        }
+    }
     if (q.vec)
         project_block<T, true>(q, it.y, P, threadIdx.x);
     else
