@@ -375,7 +375,8 @@ __global__ __launch_bounds__(64) void tsqr_gram_chol(const TreeDesc d, T* __rest
 
 // Pass 2 of the Gram path, one wavefront per superleaf.
 // Prologue: M = X S on the matrix cores (X = R^{-1} from side[leaf], S the
-// superleaf's block of the tree's Q, identity at the root).  The MFMA
+// superleaf's block of the tree's Q; at the root, and on the flat path, S
+// is the identity and M is X itself, loaded in the same layout).  The MFMA
 // accumulator layout of M (lane l, register v of tile (i0, j0) holds
 // M[i0 + 4v + l/16][j0 + l%16]) is exactly the B-operand layout of the
 // k-group k0 = i0 + 4v, so M stays in registers as mb[k-group][j-tile].
@@ -419,25 +420,33 @@ __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restric
         S = blocks + d.lv[1].blk_off + (int64_t)parent * d.H * n + slot * n;  // ld H
     }
     double mb[8][2];
+    if (root) {
+        // S = I_n: M is X with the columns past n cleared, read straight
+        // into the B-operand layout (k-group kg holds row 4 kg + l/16)
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
+        for (int kg = 0; kg < 8; ++kg)
 #pragma unroll
-        for (int jt = 0; jt < 2; ++jt) {
-            v4d acc = {0, 0, 0, 0};
-#pragma unroll
-            for (int kg = 0; kg < 8; ++kg) {
-                const int k = 4 * kg + lq;
-                const double xa = X[(16 * it + li) * 32 + k];
+            for (int jt = 0; jt < 2; ++jt) {
                 const int jc = 16 * jt + li;
-                double sb;
-                if (root)
-                    sb = (k == jc && k < n) ? 1.0 : 0.0;
-                else
-                    sb = (k < n && jc < n) ? (double)S[k + (int64_t)jc * d.H] : 0.0;
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, sb, acc, 0, 0, 0);
+                mb[kg][jt] = jc < n ? X[(4 * kg + lq) * 32 + jc] : 0.0;
             }
+    } else {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) mb[4 * it + v][jt] = acc[v];
+        for (int it = 0; it < 2; ++it) {
+#pragma unroll
+            for (int jt = 0; jt < 2; ++jt) {
+                v4d acc = {0, 0, 0, 0};
+#pragma unroll
+                for (int kg = 0; kg < 8; ++kg) {
+                    const int k = 4 * kg + lq;
+                    const double xa = X[(16 * it + li) * 32 + k];
+                    const int jc = 16 * jt + li;
+                    const double sb = (k < n && jc < n) ? (double)S[k + (int64_t)jc * d.H] : 0.0;
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, sb, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) mb[4 * it + v][jt] = acc[v];
+            }
         }
     }
     // ---- Q rows = A rows * M ----
