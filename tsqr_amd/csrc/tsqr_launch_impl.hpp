@@ -60,6 +60,12 @@ __global__ void init_flags(int* __restrict__ tickets, int nt, int* __restrict__ 
 // partials (two deterministic stages); returns the partials and their count.
 inline const double* reduce_grams(int count, const double* src, double* partial, const int* run_if, int* nparts,
                                   hipStream_t st) {
+    if (count <= kGramPartials2 * kGramPartials2) {  // one stage: at most 32 blocks per workgroup
+        const int p = count < kGramPartials2 ? count : kGramPartials2;
+        hipLaunchKernelGGL(tsqr_gram_partial<>, dim3(p), dim3(256), 0, st, count, src, partial, run_if);
+        *nparts = p;
+        return partial;
+    }
     const int p1 = count < kGramPartials ? count : kGramPartials;
     hipLaunchKernelGGL(tsqr_gram_partial<>, dim3(p1), dim3(256), 0, st, count, src, partial, run_if);
     if (p1 <= kGramPartials2) {
