@@ -101,6 +101,47 @@ __global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_r(const Tre
     }
 }
 
+// tsqr_super_r and tsqr_tree_up<FORMQ> in one launch, for the paths where
+// every superleaf is listed Householder (the flat paths' fallback): after a
+// superleaf's R is in its level-1 parent's block, wave 0 takes the parent's
+// ticket, and the last child to arrive walks up the tree from that parent
+// (tree_up_body, in the LDS of wave 0).  Nothing waits.  Needs two or more
+// tree levels.
+template <typename T, int N, int RPL, bool VEC>
+__global__ __launch_bounds__(64 * super_tiles<T>(N)) void tsqr_super_r_up(const TreeDesc d, const T* __restrict__ A,
+                                                                            int lda, T* __restrict__ R, int ldr,
+                                                                            T* __restrict__ blocks,
+                                                                            int* __restrict__ tickets,
+                                                                            const int* __restrict__ list,
+                                                                            const int* __restrict__ ctl) {
+    __shared__ SuperLds<T, N, RPL> S;
+    if (ctl[0]) return;  // the flat Gram path took the whole matrix
+    const int count = list[0];
+    const LevelDesc& L0 = d.lv[0];
+    const LevelDesc& L1 = d.lv[1];
+#pragma unroll 1
+    for (int it = blockIdx.x; it < count; it += gridDim.x) {
+        const int leaf = list[1 + it];
+        super_r_leaf<T, N, RPL, VEC>(d, A, lda, R, ldr, blocks, S, leaf);
+        if (threadIdx.x < 64) {
+            const int parent = leaf / d.arity;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            int old = 0;
+            if (threadIdx.x == 0)
+                old = __hip_atomic_fetch_add(&tickets[L1.ctr_off + parent], 1, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+            old = __builtin_amdgcn_readfirstlane(old);
+            const int left = L0.tiles - parent * d.arity;
+            const int nchild = left < d.arity ? left : d.arity;
+            if (old == nchild - 1) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                tree_up_body<T, N, RPL, true>(d, R, ldr, blocks, tickets, S.wave[0], parent);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // PATHS: the tree above holds each node's explicit Q (tsqr_tree_up with
 // FORMQ, no down sweep), and wave 0 forms this superleaf's S with path_s in
 // the LDS of waves 1 to 3, which are idle between the two barriers.

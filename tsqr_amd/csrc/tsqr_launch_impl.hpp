@@ -166,22 +166,23 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
         hipLaunchKernelGGL((tsqr_gram_chol<T>), dim3(chol_grid), dim3(64), 0, st, d, R, ldr, blocks, side, flags, list,
                            (const int*)ctl, o.gate);
     }
-    hipLaunchKernelGGL((tsqr_super_r<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st, d,
-                       (const T*)A, lda, R, ldr, blocks, (const int*)list, (const int*)ctl);
-    if (timed) timing_mark(kPhaseLeafR, false, st);
     // Where the Householder kernels only back up a flat path, the tree keeps
     // every node's explicit Q and each superleaf forms its own S (path_s), so
     // no down-sweep launches (one per level) are queued for a path that
-    // nearly never runs.
+    // nearly never runs, and the superleaf R kernel walks up the tree itself
+    // (tsqr_super_r_up).
     const bool flat_first = flat_auto || mixed;
-    if (d.levels >= 2) {
+    if (flat_first && d.levels >= 2)
+        hipLaunchKernelGGL((tsqr_super_r_up<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st,
+                           d, (const T*)A, lda, R, ldr, blocks, tickets, (const int*)list, (const int*)ctl);
+    else
+        hipLaunchKernelGGL((tsqr_super_r<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st, d,
+                           (const T*)A, lda, R, ldr, blocks, (const int*)list, (const int*)ctl);
+    if (timed) timing_mark(kPhaseLeafR, false, st);
+    if (d.levels >= 2 && !flat_first) {
         if (timed) timing_mark(kPhaseTree, true, st);
-        if (flat_first)
-            hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, true>), dim3(d.lv[1].tiles), dim3(64), 0, st, d, R, ldr,
-                               blocks, tickets, (const int*)ctl);
-        else
-            hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, false>), dim3(d.lv[1].tiles), dim3(64), 0, st, d, R, ldr,
-                               blocks, tickets, (const int*)ctl);
+        hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, false>), dim3(d.lv[1].tiles), dim3(64), 0, st, d, R, ldr, blocks,
+                           tickets, (const int*)ctl);
         if (timed) timing_mark(kPhaseTree, false, st);
     }
     if (o.r_only) return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
