@@ -40,6 +40,9 @@ namespace tsqr {
 constexpr int kFallbackGrid = 4096;
 // Waves the Q pass of the Gram paths aims for when superleaves are few.
 constexpr int kGramQWaves = 2048;
+// Most superleaves for which an AUTO call walks up the tree inside the
+// superleaf R kernel (tsqr_super_r_up).
+constexpr int kFusedWalkLeaves = 256;
 
 namespace {
 // tickets = 0; flags = value; the Householder list is empty (value 0) or
@@ -172,17 +175,26 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     // nearly never runs, and the superleaf R kernel walks up the tree itself
     // (tsqr_super_r_up).
     const bool flat_first = flat_auto || mixed;
-    if (flat_first && d.levels >= 2)
+    // The fused walk saves a launch, which matters on short AUTO calls; it
+    // measured slower on CholeskyQR2 calls and on the fallback of tall
+    // matrices (one wave walks while the rest of its workgroup idles), which
+    // keep the separate tree launch.
+    const bool fused_up = flat_auto && d.levels >= 2 && leaves <= kFusedWalkLeaves;
+    if (fused_up)
         hipLaunchKernelGGL((tsqr_super_r_up<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st,
                            d, (const T*)A, lda, R, ldr, blocks, tickets, (const int*)list, (const int*)ctl);
     else
         hipLaunchKernelGGL((tsqr_super_r<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st, d,
                            (const T*)A, lda, R, ldr, blocks, (const int*)list, (const int*)ctl);
     if (timed) timing_mark(kPhaseLeafR, false, st);
-    if (d.levels >= 2 && !flat_first) {
+    if (d.levels >= 2 && !fused_up) {
         if (timed) timing_mark(kPhaseTree, true, st);
-        hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, false>), dim3(d.lv[1].tiles), dim3(64), 0, st, d, R, ldr, blocks,
-                           tickets, (const int*)ctl);
+        if (flat_first)
+            hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, true>), dim3(d.lv[1].tiles), dim3(64), 0, st, d, R, ldr,
+                               blocks, tickets, (const int*)ctl);
+        else
+            hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, false>), dim3(d.lv[1].tiles), dim3(64), 0, st, d, R, ldr,
+                               blocks, tickets, (const int*)ctl);
         if (timed) timing_mark(kPhaseTree, false, st);
     }
     if (o.r_only) return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
