@@ -40,7 +40,7 @@ namespace tsqr {
 constexpr int kFallbackGrid = 4096;
 // Waves the Q pass of the Gram paths aims for when superleaves are few.
 constexpr int kGramQWaves = 2048;
-// Most superleaves for which an AUTO call walks up the tree inside the
+// Most superleaves for which a flat-path call walks up the tree inside the
 // superleaf R kernel (tsqr_super_r_up).
 constexpr int kFusedWalkLeaves = 256;
 
@@ -175,11 +175,10 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     // nearly never runs, and the superleaf R kernel walks up the tree itself
     // (tsqr_super_r_up).
     const bool flat_first = flat_auto || mixed;
-    // The fused walk saves a launch, which matters on short AUTO calls; it
-    // measured slower on CholeskyQR2 calls and on the fallback of tall
-    // matrices (one wave walks while the rest of its workgroup idles), which
-    // keep the separate tree launch.
-    const bool fused_up = flat_auto && d.levels >= 2 && leaves <= kFusedWalkLeaves;
+    // The fused walk saves a launch, which matters on short calls; on tall
+    // matrices it measured slower (one wave walks while the rest of its
+    // workgroup idles), and they keep the separate tree launch.
+    const bool fused_up = flat_first && d.levels >= 2 && leaves <= kFusedWalkLeaves;
     if (fused_up)
         hipLaunchKernelGGL((tsqr_super_r_up<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st,
                            d, (const T*)A, lda, R, ldr, blocks, tickets, (const int*)list, (const int*)ctl);
