@@ -31,8 +31,10 @@
  *     qr_* takes any n <= m;
  *   - the work array of the reference-shaped entry points holds ldwork*n
  *     elements; if that is smaller than tsqr_workspace_bytes(), the library
- *     uses a per-device internal buffer instead (not safe for concurrent
- *     calls on one device -- pass a large enough work array for that);
+ *     uses a per-device internal buffer instead; a call holds that buffer's
+ *     lock until its launches are enqueued on the null stream, so calls
+ *     from several host threads take turns on it (a large enough work
+ *     array, or the *_stream entry points, avoid the lock);
  *   - a zero column yields a zero R diagonal entry and an identity reflector
  *     (LAPACK dlarfg convention) where the reference divides by zero.
  */

@@ -7,8 +7,13 @@ The HIP library (tsqr_amd) never imports, links or falls back to it.
 Contents
   gen_uniform(m, n, dtype, seed)  the reference harness input stream
                                   (test_tsqr.cu:24-28), column-major
-  gen_normal(m, n, seed)          seeded N(0,1) inputs (mt19937_64 + Box-Muller
-                                  in numpy; no reference counterpart)
+  gen_normal(m, n, seed)          seeded N(0,1) inputs for the general tests
+                                  (numpy MT19937 + standard_normal; no
+                                  reference counterpart)
+  gen_normal_stream(m, n, dtype)  the recursive-QR configuration's N(0,1)
+                                  input: std::mt19937_64(0) + Box-Muller in
+                                  FP64, column-major (C, pinned by
+                                  tests/golden/normal_stream.json)
   tsqr_ref(A, bs, num_sm)         C restatement of tsqr<T> (TallShinnyQR.h:86-149,
                                   kernelQR.h:69-497), float32 or float64
   qr_ref(A, bs, num_sm, nb)       C restatement of qr<T> (block_QR.h:72-95)
@@ -57,6 +62,8 @@ def lib() -> ctypes.CDLL:
         for suf in ("s", "d"):
             getattr(L, f"oracle_gen_uniform_{suf}").argtypes = [vp, ctypes.c_int64, ctypes.c_int64, u32]
             getattr(L, f"oracle_gen_uniform_{suf}").restype = None
+            getattr(L, f"oracle_gen_normal_{suf}").argtypes = [vp, ctypes.c_int64, ctypes.c_uint64]
+            getattr(L, f"oracle_gen_normal_{suf}").restype = None
             f = getattr(L, f"oracle_qr_tile_{suf}")
             f.argtypes = [i32, i32, vp, i64, vp, i64, vp, i64]
             f.restype = None
@@ -66,6 +73,10 @@ def lib() -> ctypes.CDLL:
             f = getattr(L, f"oracle_qr_{suf}")
             f.argtypes = [i32, i32, vp, i64, vp, i64, i32, i32, i32]
             f.restype = i32
+        L.oracle_mt64_raw.argtypes = [vp, ctypes.c_int64, ctypes.c_uint64]
+        L.oracle_mt64_raw.restype = None
+        L.oracle_fnv1a64.argtypes = [vp, ctypes.c_int64]
+        L.oracle_fnv1a64.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -105,6 +116,29 @@ def gen_normal(m: int, n: int, dtype=np.float32, seed: int = 0) -> np.ndarray:
     """Seeded iid N(0,1), Fortran order (no reference counterpart)."""
     rng = np.random.Generator(np.random.MT19937(seed))
     return np.asfortranarray(rng.standard_normal((m, n)).astype(dtype))
+
+
+def mt64_raw(count: int, seed: int = 0) -> np.ndarray:
+    """The first `count` outputs of std::mt19937_64(seed) (C restatement)."""
+    out = np.empty(count, dtype=np.uint64)
+    lib().oracle_mt64_raw(_ptr(out), count, seed)
+    return out
+
+
+def fnv1a64(a: np.ndarray) -> str:
+    """FNV-1a-64 over the bytes of a contiguous array, as hex (the fixtures'
+    array hash)."""
+    a = np.ascontiguousarray(a)
+    return "%016x" % lib().oracle_fnv1a64(_ptr(a), a.nbytes)
+
+
+def gen_normal_stream(m: int, n: int, dtype=np.float32, seed: int = 0) -> np.ndarray:
+    """m x n N(0,1) matrix of the recursive-QR configuration, filled
+    column-major from std::mt19937_64(seed) by Box-Muller in FP64 (float32:
+    the FP64 values rounded); Fortran-ordered."""
+    out = np.empty(m * n, dtype=dtype)
+    getattr(lib(), f"oracle_gen_normal_{_suf(dtype)}")(_ptr(out), m * n, seed)
+    return out.reshape(n, m).T
 
 
 def gen_conditioned(m: int, n: int, cond: float, dtype=np.float32, seed: int = 0) -> np.ndarray:

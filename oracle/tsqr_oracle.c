@@ -8,6 +8,9 @@
  *
  * Contents (each function cites the reference file:line it restates):
  *   oracle_gen_uniform_{s,d}  test_tsqr.cu:24-28 input stream
+ *   oracle_gen_normal_{s,d}   N(0,1) input of the recursive-QR configuration
+ *                             (std::mt19937_64(0) + Box-Muller; no reference
+ *                             counterpart), oracle_mt64_raw its engine
  *   oracle_qr_tile_{s,d}      kernelQR.h:69-291
  *   oracle_tsqr_{s,d}         kernelQR.h:299-497 + TallShinnyQR.h:86-149
  *   oracle_qr_{s,d}           block_QR.h:72-95
@@ -111,6 +114,86 @@ void oracle_gen_uniform_d(double *out, int64_t first, int64_t count, uint32_t se
         }
     }
 }
+
+/* ---------------------------------------------------------------------------
+ * Seeded N(0,1) input of the recursive-QR configuration (no reference
+ * counterpart: the reference harness only draws uniforms).  The engine is
+ * std::mt19937_64 restated from its published definition (Matsumoto and
+ * Nishimura 2000; degree 312, middle word 156, twist constant
+ * 0xB5026F5AA96619E9, the C++ standard's seeding and tempering); the normal
+ * values are Box-Muller in FP64 over pairs of 53-bit uniforms
+ * u = (x >> 11) 2^-53: r = sqrt(-2 log(1 - u1)), t = 2 pi u2, z0 = r cos t,
+ * z1 = r sin t.  Pinned by tests/golden/normal_stream.json, written by
+ * libstdc++ and glibc themselves (tests/golden/gen_normal_golden.cpp).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    uint64_t s[312];
+    int i;
+} mt64;
+
+static void mt64_seed(mt64 *g, uint64_t seed) {
+    g->s[0] = seed;
+    for (int k = 1; k < 312; ++k)
+        g->s[k] = 6364136223846793005ull * (g->s[k - 1] ^ (g->s[k - 1] >> 62)) + (uint64_t)k;
+    g->i = 312;
+}
+
+static uint64_t mt64_next(mt64 *g) {
+    if (g->i >= 312) {
+        const uint64_t upper = 0xFFFFFFFF80000000ull, lower = 0x7FFFFFFFull;
+        for (int k = 0; k < 312; ++k) {
+            const uint64_t y = (g->s[k] & upper) | (g->s[(k + 1) % 312] & lower);
+            g->s[k] = g->s[(k + 156) % 312] ^ (y >> 1) ^ ((y & 1u) ? 0xB5026F5AA96619E9ull : 0ull);
+        }
+        g->i = 0;
+    }
+    uint64_t x = g->s[g->i++];
+    x ^= (x >> 29) & 0x5555555555555555ull;
+    x ^= (x << 17) & 0x71D67FFFEDA60000ull;
+    x ^= (x << 37) & 0xFFF7EEE000000000ull;
+    x ^= x >> 43;
+    return x;
+}
+
+/* FNV-1a-64 of `bytes` bytes (the fixtures' array hash). */
+uint64_t oracle_fnv1a64(const void *data, int64_t bytes) {
+    const unsigned char *p = (const unsigned char *)data;
+    uint64_t h = 1469598103934665603ull;
+    for (int64_t i = 0; i < bytes; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+/* out[0..count): the raw engine outputs (for the fixture check). */
+void oracle_mt64_raw(uint64_t *out, int64_t count, uint64_t seed) {
+    mt64 g;
+    mt64_seed(&g, seed);
+    for (int64_t i = 0; i < count; ++i) out[i] = mt64_next(&g);
+}
+
+static void gen_normal(double *outd, float *outf, int64_t count, uint64_t seed) {
+    mt64 g;
+    mt64_seed(&g, seed);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int64_t i = 0; i < count; i += 2) {
+        const double u1 = (double)(mt64_next(&g) >> 11) * 0x1.0p-53;
+        const double u2 = (double)(mt64_next(&g) >> 11) * 0x1.0p-53;
+        const double r = sqrt(-2.0 * log(1.0 - u1));
+        const double t = two_pi * u2;
+        const double z0 = r * cos(t), z1 = r * sin(t);
+        if (outd) outd[i] = z0;
+        if (outf) outf[i] = (float)z0;
+        if (i + 1 < count) {
+            if (outd) outd[i + 1] = z1;
+            if (outf) outf[i + 1] = (float)z1;
+        }
+    }
+}
+
+void oracle_gen_normal_s(float *out, int64_t count, uint64_t seed) { gen_normal(NULL, out, count, seed); }
+void oracle_gen_normal_d(double *out, int64_t count, uint64_t seed) { gen_normal(out, NULL, count, seed); }
 
 #define REAL float
 #define SUF s

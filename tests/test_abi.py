@@ -87,3 +87,62 @@ def test_compat_header_compiles_reference_style_caller(tmp_path):
     r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-I", inc, "-c", str(src), "-o",
                         str(tmp_path / "caller.o")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_reference_ldwork_short_at_262144(lib):
+    """The drop-in harness case 262144 x 32 float really takes the library's
+    internal buffer: the reference-sized work array (2 NUM_SM BLOCK_SIZE rows
+    of n elements, NUM_SM = BLOCK_SIZE = 256 in tsqr_compat.hpp) is smaller
+    than the workspace."""
+    m, n = 262144, 32
+    assert lib.tsqr_workspace_bytes(m, n, 1) > 2 * 256 * 256 * n * 4
+
+
+def panel_widths(n):
+    """Widths of the TSQR panels the recursive qr factors: the column split
+    n1 = 32 ceil(n / 64), n2 = n - n1 applied until n <= 32."""
+    if n <= 32:
+        return [n]
+    n1 = 32 * ((n + 63) // 64)
+    return panel_widths(n1) + panel_widths(n - n1)
+
+
+def _qr_covers_panels(lib, ms, ns):
+    bad = []
+    for dtype in (1, 2):
+        for m in ms:
+            for n in ns:
+                if n > m:
+                    continue
+                q = lib.qr_workspace_bytes(m, n, dtype)
+                for w in set(panel_widths(n)):
+                    t = lib.tsqr_workspace_bytes(m, w, dtype)
+                    if q < t:
+                        bad.append((dtype, m, n, w, q, t))
+    return bad
+
+
+MS = [1000, 4096, 27648, 100000, 262144, 300000, 1 << 20, 3000000]
+NS = [33, 40, 48, 64, 72, 96, 100, 128, 200, 256]
+
+
+def test_qr_workspace_covers_every_panel(lib):
+    """qr_workspace_bytes holds the TSQR workspace of every panel width the
+    recursion produces: narrow panels (a 16-wide last panel of n = 48) can
+    need more than a 32-wide one when they get more superleaves."""
+    assert not _qr_covers_panels(lib, MS, NS)
+
+
+def test_qr_workspace_covers_every_panel_tuned(lib):
+    """Same under the measured tuning table, which gives narrower panels
+    more, shorter superleaves (262144 x 48 FP64 needed 16.4 MB for the panel
+    region and had 19.3 MB worth of writes)."""
+    from tsqr_amd.autotune import DEFAULT_FILE, apply, load, reset
+
+    table = load(DEFAULT_FILE)
+    try:
+        apply(table)
+        assert lib.qr_workspace_bytes(262144, 48, 2) >= lib.tsqr_workspace_bytes(262144, 16, 2)
+        assert not _qr_covers_panels(lib, MS, NS)
+    finally:
+        reset()

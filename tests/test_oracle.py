@@ -55,6 +55,55 @@ def test_generator_first_values_cfg1(golden):
     assert np.array_equal(full[4000:], tail)
 
 
+def test_generator_full_hash_cfg1(golden):
+    """The whole 27648 x 32 float stream and the 27648 x 32 double stream of
+    the harness (test_tsqr.cu:24-28) hash to libstdc++'s."""
+    for name, dtype in (("float_27648x32", np.float32), ("double_27648x32", np.float64)):
+        e = golden[name]
+        a = O.gen_uniform_flat(e["count"], dtype)
+        assert O.fnv1a64(a) == e["fnv1a64"], name
+    # the C hash agrees with the plain definition
+    a = O.gen_uniform_flat(4096, np.float32)
+    assert O.fnv1a64(a) == fnv1a64(a.tobytes())
+
+
+NORMAL = os.path.join(os.path.dirname(__file__), "golden", "normal_stream.json")
+
+
+@pytest.fixture(scope="module")
+def normal_golden():
+    with open(NORMAL) as f:
+        return json.load(f)
+
+
+def test_mt19937_64_matches_libstdcxx(normal_golden):
+    assert ["%016x" % v for v in O.mt64_raw(8, 0)] == normal_golden["mt19937_64_first8_hex"]
+
+
+def test_normal_stream_matches_fixture(normal_golden):
+    """The recursive-QR configuration's 27648 x 128 N(0,1) input: first
+    values and the hash of the whole FP64 and FP32 arrays (column-major)."""
+    A = O.gen_normal_stream(27648, 128, np.float64)
+    flat = A.ravel(order="F")
+    assert ["%016x" % v for v in flat[:16].view(np.uint64)] == normal_golden["normal_first16_hex"]
+    assert O.fnv1a64(flat) == normal_golden["double_27648x128"]["fnv1a64"]
+    Af = O.gen_normal_stream(27648, 128, np.float32)
+    assert O.fnv1a64(Af.ravel(order="F")) == normal_golden["float_27648x128"]["fnv1a64"]
+    assert np.array_equal(Af, A.astype(np.float32))
+    # N(0,1) to sampling accuracy
+    assert abs(flat.mean()) < 5e-3 and abs(flat.std() - 1.0) < 5e-3
+
+
+def test_batch_shapes_match_fixture(normal_golden):
+    """The batched configuration's 512 (m, n) pairs (tsqr_amd/workloads.py)
+    are libstdc++'s, and obey the stated distribution's support."""
+    from tsqr_amd.workloads import batch_shapes
+
+    shapes = batch_shapes()
+    assert shapes == [tuple(x) for x in normal_golden["batch_512_shapes"]]
+    assert all(n in (8, 16, 32, 64) and m % n == 0 and 4096 - n < m <= 262144 for m, n in shapes)
+
+
 def test_reflector_kat_3_4():
     """x = (3, 4): R00 = -5, Q = -(0.6, 0.8) (u = (1.26491, 0.63246))."""
     Q, R = O.qr_tile_ref(np.array([[3.0], [4.0]]))

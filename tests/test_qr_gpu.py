@@ -5,7 +5,8 @@ Gram-Schmidt (block_QR.h:88-93), so orthogonality between the halves degrades
 like u*kappa(A); the inputs here are N(0,1) (kappa ~ 1.2 for m >> n).  Bars:
 backward and orthogonality <= 8 n u * kappa, R vs FP64 LAPACK after sign
 normalisation <= 8 n u * kappa^2, R21 exactly zero, and R12 = Q1^T A2 against
-an FP64 product of the returned Q1 and the input A2.
+an FP64 product of the returned Q1 and the input A2.  The 27648 x 128
+configuration uses its pinned N(0,1) input (oracle.gen_normal_stream).
 """
 import numpy as np
 import pytest
@@ -35,7 +36,9 @@ def check_qr(A, Q, R, tag):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_cfg2_27648x128(dtype):
-    A = O.gen_normal(27648, 128, dtype, seed=0)
+    # the configuration's pinned input: std::mt19937_64(0) + Box-Muller
+    # (tests/golden/normal_stream.json)
+    A = O.gen_normal_stream(27648, 128, dtype, seed=0)
     st, Q, R, _ = call("qr", A)
     assert st == 0
     check_qr(A, Q, R, f"qr 27648x128 {np.dtype(dtype).name}")

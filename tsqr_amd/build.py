@@ -65,7 +65,22 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
+    build_harness(verbose)
     return LIB
+
+
+def build_harness(verbose: bool = False) -> str:
+    """Builds tests/dropin/ref_harness: a caller written like the reference's
+    test_tsqr.cu / test_qr.cu against include/tsqr_compat.hpp, linked with
+    -ltsqr_amd (host code only)."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "dropin", "ref_harness.cpp")
+    out = os.path.join(root, "tests", "dropin", "ref_harness")
+    tmp = out + ".tmp"
+    _run([hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"),
+          src, "-L", HERE, "-ltsqr_amd", "-Wl,-rpath,$ORIGIN/../../tsqr_amd", "-o", tmp], verbose)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

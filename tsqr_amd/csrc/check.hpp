@@ -36,13 +36,16 @@ namespace tsqr {
 // rs_j = 1) this is the Gram matrix of A R1^{-1}, the second CholeskyQR pass
 // of the R-only entry points, formed without writing the product.  R is
 // read as R[k rs_k + j rs_j] (column-major: rs_k = 1, rs_j = ldr).
+// If run_if is given and run_if[0] == 0 (the flat path was not taken, so R
+// is not written), every workgroup returns at once and nothing is written.
 template <typename T, typename TR, int NT, bool VEC, bool RESID>
 __global__ __launch_bounds__(256) void tsqr_resid_stream(int m, int n, const T* __restrict__ A, int lda,
                                                           const T* __restrict__ Q, int ldq, const TR* __restrict__ R,
                                                           int rs_k, int rs_j, double* __restrict__ partial,
-                                                          int* __restrict__ tk) {
+                                                          int* __restrict__ tk, const int* __restrict__ run_if) {
     __shared__ GramLds L;
     __shared__ int bcast;
+    if (run_if && !run_if[0]) return;
     const int lane = threadIdx.x & 63;
     const int li = lane & 15, lq = lane >> 4;
     const int P = gridDim.x;
@@ -133,7 +136,7 @@ int check_launch_n(int m, int n, const T* A, int lda, const T* Q, int ldq, const
     hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(P), dim3(256), 0, st, d, Q, ldq, base + part,
                        tk + kCtlStreamTickets);
     hipLaunchKernelGGL((tsqr_resid_stream<T, T, NT, VEC, true>), dim3(P), dim3(256), 0, st, m, n, A, lda, Q, ldq, R,
-                       1, ldr, base + 2 * part, tk + 2 * kCtlStreamTickets);
+                       1, ldr, base + 2 * part, tk + 2 * kCtlStreamTickets, (const int*)nullptr);
     for (int k = 0; k < 3; ++k) offsets[k] = (k * part + (size_t)P * 1024) * sizeof(double);
     return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
 }

@@ -76,16 +76,26 @@ struct Pool {
     void* ptr = nullptr;
     size_t bytes = 0;
 };
-std::mutex g_pool_mu;
-Pool g_pool[64];        // the reference-shaped entry points
-Pool g_check_pool[64];  // the device-side checker
+// Per-device internal workspaces: one for the reference-shaped entry points
+// (whose ldwork is short), one for the device-side checker.  A caller holds
+// the pool's mutex from the moment it takes the buffer until its launches
+// are enqueued (the checker: until its result is back), so concurrent calls
+// from several host threads take turns instead of sharing the tickets and
+// partial sums, and a buffer is never freed while a call still holds it.
+constexpr int kMaxDevices = 64;
+std::mutex g_pool_mu[kMaxDevices];
+std::mutex g_check_mu[kMaxDevices];
+Pool g_pool[kMaxDevices];
+Pool g_check_pool[kMaxDevices];
 
-// Per-device internal workspace (grows, never shrinks).
-int pool_get(size_t bytes, void** out, Pool* pools = g_pool) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return TSQR_ERR_HIP;
-    std::lock_guard<std::mutex> lock(g_pool_mu);
-    Pool& p = pools[dev];
+int current_device(int* dev) {
+    if (hipGetDevice(dev) != hipSuccess || *dev < 0 || *dev >= kMaxDevices) return TSQR_ERR_HIP;
+    return TSQR_OK;
+}
+
+// Grows p to at least `bytes` (never shrinks); the caller holds p's mutex.
+// Work enqueued earlier on the old buffer is waited for before it is freed.
+int pool_reserve(Pool& p, size_t bytes, void** out) {
     if (p.bytes < bytes) {
         if (p.ptr) {
             if (hipDeviceSynchronize() != hipSuccess) return TSQR_ERR_HIP;
@@ -205,13 +215,25 @@ inline size_t tn_partial_elems(int m, int n) {
     return mx > b ? mx : b;
 }
 
+// Largest TSQR workspace over the panels the recursion factors (widths
+// between 1 and 32).  Narrow panels can need more than a 32-wide one: the
+// tuned superleaf height depends on the padded width.
+template <typename T>
+size_t panel_ws_bytes(int m, int n) {
+    if (n <= 32) return tsqr_ws_bytes<T>(m, n);
+    const int n1 = split_cols(n);
+    const size_t a = panel_ws_bytes<T>(m, n1), b = panel_ws_bytes<T>(m, n - n1);
+    return a > b ? a : b;
+}
+
 // Panel TSQR workspace, then the split-K partials, then n1 x n2 elements for
 // the mixed method's second coupling product.
 template <typename T>
 size_t qr_ws_bytes(int m, int n) {
     if (n < 1 || m < n) return 0;
     if (n <= 32) return tsqr_ws_bytes<T>(m, n);
-    const size_t panel = tsqr_ws_bytes<T>(m, 32);
+    const size_t panel = panel_ws_bytes<T>(m, n);
+    if (panel == 0) return 0;
     const int n1 = split_cols(n);
     return align256(panel) + align256(tn_partial_elems(m, n) * sizeof(T)) +
            align256((size_t)n1 * (n - n1) * sizeof(T));
@@ -314,7 +336,7 @@ int qr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t wor
     if (need == 0) return TSQR_ERR_SHAPE;
     if (work_bytes < need || !work) return TSQR_ERR_WORKSPACE;
     if (n <= 32) return tsqr_core<T>(m, n, A, lda, R, ldr, work, st, mixed);
-    const size_t panel = align256(tsqr_ws_bytes<T>(m, 32));
+    const size_t panel = align256(panel_ws_bytes<T>(m, n));
     const size_t parts = align256(tn_partial_elems(m, n) * sizeof(T));
     char* ws = reinterpret_cast<char*>(work);
     // every R21 block of the recursion at once (the panels write their own
@@ -363,8 +385,13 @@ int check_impl(int m, int n, const T* A, int lda, const T* Q, int ldq, const T* 
                hipStream_t st) {
     if (!A || !Q || !R || !norms || n < 1 || m < n || lda < m || ldq < m || ldr < n) return TSQR_ERR_ARG;
     if (n > 32) return TSQR_ERR_SHAPE;
+    int dev = 0;
+    int e = current_device(&dev);
+    if (e) return e;
+    // the pool is held until the sums have been copied back
+    std::lock_guard<std::mutex> lock(g_check_mu[dev]);
     void* work = nullptr;
-    int e = pool_get(check_workspace_bytes(m), &work, g_check_pool);
+    e = pool_reserve(g_check_pool[dev], check_workspace_bytes(m), &work);
     if (e) return e;
     size_t off[3];
     e = check_launch<T>(m, n, A, lda, Q, ldq, R, ldr, work, off, st);
@@ -525,16 +552,22 @@ int apply_qt_stream(int m, int n, int k, const T* A, int lda, const T* R, int ld
     return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
 }
 
-template <typename T>
-int with_ref_workspace(int m, int n, T* work, int ldwork, size_t need, void** ws, size_t* bytes) {
+// Runs fn(ws, bytes) on the caller's work array of the reference-shaped
+// entry points (ldwork * n elements, test_tsqr.cu:42) if it holds `need`
+// bytes at a 256-byte aligned address, else on the per-device pool, whose
+// mutex is held until fn has enqueued its launches.
+template <typename T, typename F>
+int with_ref_workspace(int n, T* work, int ldwork, size_t need, F&& fn) {
     const size_t have = (work && ldwork > 0) ? (size_t)ldwork * (size_t)n * sizeof(T) : 0;
-    if (have >= need && ((uintptr_t)work % 256) == 0) {
-        *ws = work;
-        *bytes = have;
-        return TSQR_OK;
-    }
-    *bytes = need;
-    return pool_get(need, ws);
+    if (have >= need && ((uintptr_t)work % 256) == 0) return fn((void*)work, have);
+    int dev = 0;
+    int e = current_device(&dev);
+    if (e) return e;
+    std::lock_guard<std::mutex> lock(g_pool_mu[dev]);
+    void* ws = nullptr;
+    e = pool_reserve(g_pool[dev], need, &ws);
+    if (e) return e;
+    return fn(ws, need);
 }
 
 template <typename T>
@@ -543,11 +576,9 @@ int tsqr_r_ref(int m, int n, const T* A, int lda, T* R, int ldr, T* work, int ld
     if (e) return e;
     const size_t need = tsqr_ws_bytes<T>(m, n);
     if (need == 0) return TSQR_ERR_SHAPE;
-    void* ws = nullptr;
-    size_t bytes = 0;
-    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
-    if (e) return e;
-    return tsqr_stream<T>(m, n, const_cast<T*>(A), lda, R, ldr, ws, bytes, nullptr, false, true);
+    return with_ref_workspace(n, work, ldwork, need, [&](void* ws, size_t bytes) {
+        return tsqr_stream<T>(m, n, const_cast<T*>(A), lda, R, ldr, ws, bytes, nullptr, false, true);
+    });
 }
 
 }  // namespace
@@ -804,66 +835,54 @@ int tsqr_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int 
     if (e) return e;
     const size_t need = tsqr_ws_bytes<float>(m, n);
     if (need == 0) return TSQR_ERR_SHAPE;
-    void* ws = nullptr;
-    size_t bytes = 0;
-    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
-    if (e) return e;
-    return tsqr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+    return with_ref_workspace(n, work, ldwork, need, [&](void* ws, size_t bytes) {
+        return tsqr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+    });
 }
 int tsqr_mixed_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int ldwork) {
     int e = check_args(m, n, A, lda, R, ldr);
     if (e) return e;
     const size_t need = tsqr_ws_bytes<float>(m, n);
     if (need == 0) return TSQR_ERR_SHAPE;
-    void* ws = nullptr;
-    size_t bytes = 0;
-    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
-    if (e) return e;
-    return tsqr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr, true);
+    return with_ref_workspace(n, work, ldwork, need, [&](void* ws, size_t bytes) {
+        return tsqr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr, true);
+    });
 }
 int tsqr_d(int m, int n, double* A, int lda, double* R, int ldr, double* work, int ldwork) {
     int e = check_args(m, n, A, lda, R, ldr);
     if (e) return e;
     const size_t need = tsqr_ws_bytes<double>(m, n);
     if (need == 0) return TSQR_ERR_SHAPE;
-    void* ws = nullptr;
-    size_t bytes = 0;
-    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
-    if (e) return e;
-    return tsqr_stream<double>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+    return with_ref_workspace(n, work, ldwork, need, [&](void* ws, size_t bytes) {
+        return tsqr_stream<double>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+    });
 }
 int qr_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int ldwork) {
     int e = check_args(m, n, A, lda, R, ldr);
     if (e) return e;
     const size_t need = qr_ws_bytes<float>(m, n);
     if (need == 0) return TSQR_ERR_SHAPE;
-    void* ws = nullptr;
-    size_t bytes = 0;
-    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
-    if (e) return e;
-    return qr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+    return with_ref_workspace(n, work, ldwork, need, [&](void* ws, size_t bytes) {
+        return qr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+    });
 }
 int qr_mixed_s(int m, int n, float* A, int lda, float* R, int ldr, float* work, int ldwork) {
     int e = check_args(m, n, A, lda, R, ldr);
     if (e) return e;
     const size_t need = qr_ws_bytes<float>(m, n);
     if (need == 0) return TSQR_ERR_SHAPE;
-    void* ws = nullptr;
-    size_t bytes = 0;
-    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
-    if (e) return e;
-    return qr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr, true);
+    return with_ref_workspace(n, work, ldwork, need, [&](void* ws, size_t bytes) {
+        return qr_stream<float>(m, n, A, lda, R, ldr, ws, bytes, nullptr, true);
+    });
 }
 int qr_d(int m, int n, double* A, int lda, double* R, int ldr, double* work, int ldwork) {
     int e = check_args(m, n, A, lda, R, ldr);
     if (e) return e;
     const size_t need = qr_ws_bytes<double>(m, n);
     if (need == 0) return TSQR_ERR_SHAPE;
-    void* ws = nullptr;
-    size_t bytes = 0;
-    e = with_ref_workspace(m, n, work, ldwork, need, &ws, &bytes);
-    if (e) return e;
-    return qr_stream<double>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+    return with_ref_workspace(n, work, ldwork, need, [&](void* ws, size_t bytes) {
+        return qr_stream<double>(m, n, A, lda, R, ldr, ws, bytes, nullptr);
+    });
 }
 
 }  // extern "C"

@@ -149,7 +149,7 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
                 // (R = R1, or R2 R1 with R2 = chol(Q1^T Q1))
                 hipLaunchKernelGGL((tsqr_resid_stream<T, double, NT, VEC, false>), dim3(sgroups), dim3(256), 0, st,
                                    d.m, d.n, (const T*)nullptr, 0, (const T*)A, lda, (const double*)X0, 32, 1,
-                                   partial, ctl + kCtlStreamTickets2);
+                                   partial, ctl + kCtlStreamTickets2, (const int*)ctl);
                 hipLaunchKernelGGL(tsqr_mixed_check<T>, dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
                                    (const double*)stream_sum, R, ldr, side, ctl, 1, o.tol);
             }
