@@ -54,6 +54,7 @@ extern "C" {
 #define TSQR_ERR_WORKSPACE (-3) /* explicit workspace smaller than required */
 #define TSQR_ERR_HIP (-4)       /* a HIP runtime call or kernel launch failed */
 #define TSQR_ERR_ALLOC (-5)     /* internal workspace allocation failed */
+#define TSQR_ERR_SCRATCH (-6)   /* tsqr_s_stream_noalloc: workspace above its fixed 16 MiB buffer */
 
 /* dtype codes follow the reference harness CLI (test_tsqr.cu:118-128). */
 #define TSQR_FLOAT 1
@@ -88,6 +89,20 @@ int tsqr_s_stream(int m, int n, float *A, int lda, float *R, int ldr, void *work
                   void *stream);
 int tsqr_d_stream(int m, int n, double *A, int lda, double *R, int ldr, void *work, size_t work_bytes,
                   void *stream);
+
+/* tsqr_s_stream without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.  The
+ * workspace is a fixed 16 MiB array of the library in the memory of each GPU
+ * it runs on (held until the process ends), so the call allocates nothing
+ * and calls no allocating routine; problems whose tsqr_workspace_bytes(m, n,
+ * TSQR_FLOAT) exceeds 16 MiB are rejected with TSQR_ERR_SCRATCH (n = 32:
+ * up to about 200,000 rows).  Same kernels and results as tsqr_s_stream.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (several streams or host threads at once) race and must be ordered by
+ * the caller.  The first call on each device looks up the array's device
+ * address (hipGetSymbolAddress), so make it outside graph capture. */
+int tsqr_s_stream_noalloc(int m, int n, float *A, int lda, float *R, int ldr, void *stream);
 
 /* Recursive blocked QR, replaces qr<float>/qr<double> (block_QR.h:72-95):
  * n <= 32 is one tsqr; otherwise the columns are split in two, the halves

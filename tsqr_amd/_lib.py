@@ -8,6 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtsqr_amd.so")
 
 TSQR_OK = 0
+TSQR_ERR_SCRATCH = -6
 TSQR_FLOAT = 1
 TSQR_DOUBLE = 2
 TSQR_LEAF_AUTO = 0
@@ -26,6 +27,7 @@ EXPORTS = (
     "tsqr_d",
     "tsqr_s_stream",
     "tsqr_d_stream",
+    "tsqr_s_stream_noalloc",
     "qr_s",
     "qr_d",
     "qr_s_stream",
@@ -99,6 +101,8 @@ def lib() -> ctypes.CDLL:
         f = getattr(L, name)
         f.argtypes = [i32, i32, vp, i32, vp, i32, vp, sz, vp]
         f.restype = i32
+    L.tsqr_s_stream_noalloc.argtypes = [i32, i32, vp, i32, vp, i32, vp]
+    L.tsqr_s_stream_noalloc.restype = i32
     L.tsqr_set_leaf_path.argtypes = [i32, i32, ctypes.c_double]
     L.tsqr_set_leaf_path.restype = i32
     L.tsqr_leaf_stats.argtypes = [vp, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]

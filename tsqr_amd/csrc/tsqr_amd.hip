@@ -155,12 +155,44 @@ LaunchOpts mixed_opts() {
 // Runs one TSQR; work must hold tsqr_ws_bytes<T>(m, n) bytes.
 template <typename T>
 int tsqr_core(int m, int n, T* A, int lda, T* R, int ldr, void* work, hipStream_t st, bool mixed = false,
-              bool r_only = false) {
+              bool r_only = false, bool untimed = false) {
     TreeDesc d;
     if (!plan_tsqr<T>(m, n, &d)) return TSQR_ERR_SHAPE;
     LaunchOpts o = mixed ? mixed_opts() : current_opts<T>();
     o.r_only = r_only ? 1 : 0;
+    o.untimed = untimed ? 1 : 0;
     return tsqr_launch<T>(d, A, lda, R, ldr, work, o, st);
+}
+
+// ------------------------------------------------------------------------
+// The entry point that allocates nothing (tsqr_s_stream_noalloc).  Its
+// workspace is `scratch`, a fixed array of this library in the memory of
+// every GPU it runs on, held until the process ends.  In host code the
+// array's name only stands in for the symbol: the device address comes from
+// hipGetSymbolAddress, looked up once per device on the first call with
+// that device current, and only that address is handed to kernels.  The
+// launches clear every ticket, flag and control word they use first
+// (init_flags), so the contents left by the previous call do not matter;
+// calls that overlap on one GPU share the array and race.
+// ------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+
+int scratch_address(void** out) {
+    int dev = 0;
+    int e = current_device(&dev);
+    if (e) return e;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TSQR_ERR_HIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TSQR_OK;
 }
 
 template <typename T>
@@ -602,6 +634,8 @@ const char* tsqr_status_string(int status) {
             return "HIP runtime error";
         case TSQR_ERR_ALLOC:
             return "internal workspace allocation failed";
+        case TSQR_ERR_SCRATCH:
+            return "problem needs more workspace than the 16 MiB fixed scratch buffer of tsqr_s_stream_noalloc";
         default:
             return "unknown status";
     }
@@ -798,6 +832,18 @@ int tsqr_s_stream(int m, int n, float* A, int lda, float* R, int ldr, void* work
 int tsqr_d_stream(int m, int n, double* A, int lda, double* R, int ldr, void* work, size_t work_bytes,
                   void* stream) {
     return tsqr_stream<double>(m, n, A, lda, R, ldr, work, work_bytes, (hipStream_t)stream);
+}
+int tsqr_s_stream_noalloc(int m, int n, float* A, int lda, float* R, int ldr, void* stream) {
+    int e = check_args(m, n, A, lda, R, ldr);
+    if (e) return e;
+    if (n > 32) return TSQR_ERR_SHAPE;
+    const size_t need = tsqr_ws_bytes<float>(m, n);
+    if (need == 0) return TSQR_ERR_SHAPE;
+    if (need > kScratchBytes) return TSQR_ERR_SCRATCH;
+    void* ws = nullptr;
+    e = scratch_address(&ws);
+    if (e) return e;
+    return tsqr_core<float>(m, n, A, lda, R, ldr, ws, (hipStream_t)stream, false, false, true);
 }
 int tsqr_mixed_s_stream(int m, int n, float* A, int lda, float* R, int ldr, void* work, size_t work_bytes,
                         void* stream) {

@@ -73,6 +73,7 @@ struct LaunchOpts {
     double gate;     // largest accepted ||R||_F ||R^{-1}||_F on the Gram path
     double tol;      // mixed path: largest accepted ||Q^T Q - I||_F
     int r_only = 0;  // R only: A is read, never written (tsqr_r_*)
+    int untimed = 0;  // record no phase events even while timing is enabled
 };
 
 // Optional per-phase timing (tsqr_timing_* in the C ABI).  When enabled,

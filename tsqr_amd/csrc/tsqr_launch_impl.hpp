@@ -87,7 +87,7 @@ template <typename T, int N, bool VEC>
 int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, const LaunchOpts& o,
                   hipStream_t st) {
     constexpr int RPL = rows_per_lane<T>(N);
-    const bool timed = timing_enabled();
+    const bool timed = timing_enabled() && !o.untimed;
     if (d.lv[0].rows <= d.H) {
         if (timed) timing_mark(kPhaseLeafR, true, st);
         hipLaunchKernelGGL((tsqr_single<T, N, RPL, VEC>), dim3(1), dim3(64), 0, st, d, A, lda, R, ldr,
