@@ -132,9 +132,14 @@ int tsqr_mixed_s_stream(int m, int n, float *A, int lda, float *R, int ldr, void
  * reference.  The same column split as qr_s; every panel of <= 32 columns is
  * tsqr_mixed_s, and the coupling of the column blocks is applied twice
  * (block classical Gram-Schmidt with reorthogonalisation): R12 = Q1^T A2,
- * A2 -= Q1 R12, then C = Q1^T A2, A2 -= Q1 C, R12 += C (FP32 matrix-core
- * products), which keeps the blocks orthogonal to each other independently of
- * cond(A).  Workspace: qr_workspace_bytes(m, n, TSQR_FLOAT). */
+ * A2 -= Q1 R12, then C = Q1^T A2, A2 -= Q1 C, R12 += C (BF16x3 matrix-core
+ * products, FP64 sums for Q1^T A2).  For 32 < n <= 128 a final CholeskyQR
+ * pass over the whole Q follows: G = Q^T Q (FP64 sums), and if
+ * ||G - I||_F exceeds the mixed tolerance, Q <- Q chol(G)^{-1} and
+ * R <- chol(G) R; this removes the loss of orthogonality between column
+ * blocks that grows with cond(R22) times the coupling products' error, so
+ * ||I - Q^T Q||_F stays below the tolerance independently of cond(A).
+ * Workspace: qr_workspace_bytes(m, n, TSQR_FLOAT). */
 int qr_mixed_s(int m, int n, float *A, int lda, float *R, int ldr, float *work, int ldwork);
 int qr_mixed_s_stream(int m, int n, float *A, int lda, float *R, int ldr, void *work, size_t work_bytes,
                       void *stream);

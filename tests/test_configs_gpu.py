@@ -111,15 +111,8 @@ def _errors64(A32, Q, R):
 MIXED_BAR = 1e-6
 
 
-# Measured before the whole-matrix reorthogonalisation pass existed: the
-# orthogonality between column blocks was 7.4e-7 .. 2.0e-5 at cond >= 1e4
-# (coupling products accumulated in FP32, amplified by cond(R22)).
-_COND_MISS = pytest.mark.xfail(reason="cross-block orthogonality above 1e-6 at cond >= 1e4 without the final "
-                                      "reorthogonalisation pass", strict=False)
-
-
 @pytest.mark.parametrize("n", [64, 128])
-@pytest.mark.parametrize("cond", [1e2, pytest.param(1e4, marks=_COND_MISS), pytest.param(1e6, marks=_COND_MISS)])
+@pytest.mark.parametrize("cond", [1e2, 1e4, 1e6])
 def test_mixed_recursive_qr_1m(n, cond):
     import torch
 

@@ -4,8 +4,10 @@ configuration (A = U diag(s) V^T, s geometric from 1 to 1/cond) at a
 reduced height.
 
 Bar (kappa-independent, which is the point of the reorthogonalised
-coupling): backward ||A-QR||_F/||A||_F <= 8 n u32 and ||I-Q^TQ||_F <= 8 n u32,
-R upper triangular, all_close(QR, A) (utils.h:352-357).  The plain qr_s, one
+coupling and the final whole-matrix CholeskyQR pass): backward
+||A-QR||_F/||A||_F <= 8 n u32 and ||I-Q^TQ||_F <= 1e-6 (the mixed-precision
+configuration's target), R upper triangular, all_close(QR, A)
+(utils.h:352-357).  The plain qr_s, one
 coupling pass as in the reference (block_QR.h:88-93), is run beside it: its
 loss of orthogonality grows with cond(A), which the sweep records.
 """
@@ -41,7 +43,7 @@ def test_qr_mixed_cond_sweep(cond, n):
     bound = 8 * n * U32
     assert mm["all_close"] and mm["r_lower_zero"]
     assert mm["backward"] <= bound
-    assert mm["orth"] <= bound
+    assert mm["orth"] <= 1e-6
     assert mp["backward"] <= bound  # the plain path stays backward stable
 
 

@@ -27,6 +27,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "bf16x3.hpp"
 #include "wave.hpp"
 
@@ -177,19 +179,27 @@ __global__ __launch_bounds__(64) void gemm_tn_mfma(int m, int n1, int n2, const 
 }
 
 // C[i + j*ldc] = (accumulate ? C : 0) + sum over chunks of
-// part[k][i + j*n1].  256 threads per 16 outputs: thread (o, g) adds the
-// chunks k = g, g + 16, ... of output o in order, then the 16 group sums
-// are added in order of g, so the result is deterministic and no thread
-// walks more than chunks / 16 partials.  Eight partials are loaded before
-// they are added (the same order), so that the loads overlap.
-template <typename T>
+// part[k][i + j*n1], summed and written in TO (T, or double for the
+// reorthogonalisation Gram matrix).  256 threads per 16 outputs: thread
+// (o, g) adds the chunks k = g, g + 16, ... of output o in order, then the
+// 16 group sums are added in order of g, so the result is deterministic and
+// no thread walks more than chunks / 16 partials.  Eight partials are
+// loaded before they are added (the same order), so that the loads overlap.
+// sym: the partials hold only the 64 x 64 tiles on and above the diagonal
+// (gemm_tn_bf16x3 with sym), and an entry of a tile below it is read from
+// the transposed position.
+template <typename T, typename TO = T>
 __global__ __launch_bounds__(256) void gemm_tn_reduce(int n1, int n2, int chunks, const T* __restrict__ part,
-                                                      T* __restrict__ C, int64_t ldc, int accumulate) {
-    __shared__ T red[16][17];
+                                                      TO* __restrict__ C, int64_t ldc, int accumulate, int sym = 0) {
+    // sums in FP64 whenever the partials or the output are FP64
+    typedef typename std::conditional<sizeof(T) == 8 || sizeof(TO) == 8, double, T>::type TS;
+    __shared__ TS red[16][17];
     const int o = threadIdx.x & 15, g = threadIdx.x >> 4;
-    const int idx = blockIdx.x * 16 + o;
+    const int out = blockIdx.x * 16 + o;
     const int total = n1 * n2;
-    T s = T(0);
+    int idx = out;
+    if (sym && out < total && (out % n1) / 64 > (out / n1) / 64) idx = (out / n1) + (out % n1) * n1;
+    TS s = TS(0);
     if (idx < total) {
         int k = g;
 #pragma unroll 1
@@ -198,19 +208,19 @@ __global__ __launch_bounds__(256) void gemm_tn_reduce(int n1, int n2, int chunks
 #pragma unroll
             for (int q = 0; q < 8; ++q) v[q] = part[(int64_t)(k + 16 * q) * total + idx];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) s += v[q];
+            for (int q = 0; q < 8; ++q) s += (TS)v[q];
         }
-        for (; k < chunks; k += 16) s += part[(int64_t)k * total + idx];
+        for (; k < chunks; k += 16) s += (TS)part[(int64_t)k * total + idx];
     }
     red[o][g] = s;
     __syncthreads();
-    if (g != 0 || idx >= total) return;
-    T sum = T(0);
+    if (g != 0 || out >= total) return;
+    TS sum = TS(0);
 #pragma unroll
     for (int q = 0; q < 16; ++q) sum += red[o][q];
-    const int i = idx % n1, j = idx / n1;
-    T* c = C + i + (int64_t)j * ldc;
-    *c = accumulate ? *c + sum : sum;
+    const int i = out % n1, j = out / n1;
+    TO* c = C + i + (int64_t)j * ldc;
+    *c = (TO)(accumulate ? (TS)*c + sum : sum);
 }
 
 // C[rows, j0:j0+64] -= X[rows, 0:K] B[0:K, j0:j0+64] for the 64-row block
@@ -291,28 +301,50 @@ __device__ __forceinline__ void load_eight(float (&x)[8], const float* __restric
 // part[chunk] for the 64 x 64 tile (blockIdx.y, blockIdx.z) of Q^T A: lane
 // (li, lq) feeds rows 8 lq .. 8 lq + 7 of each 32-row step, column li of the
 // tile's 16-column groups, as both operands.
-template <bool VEC>
+// sym (Q == A, n1 == n2, the Gram matrix Q^T Q): tiles below the diagonal
+// are not computed (gemm_tn_reduce mirrors them) and a diagonal tile loads
+// and splits its 64 columns once for both operands.
+// TP = double (the reorthogonalisation Gram matrix): each 32-row step is
+// summed in FP32 from zero and added to FP64 sums, and the partials are
+// FP64.  Long FP32 sums of positive terms (the diagonal of a Gram matrix)
+// drift low on the matrix cores -- about 5e-7 relative over 1024 rows was
+// measured on MI355X -- which the per-step FP64 sums avoid.
+template <bool VEC, typename TP = float>
 __global__ __launch_bounds__(64) void gemm_tn_bf16x3(int m, int n1, int n2, const float* __restrict__ Q, int64_t ldq,
                                                      const float* __restrict__ A, int64_t lda, int kc,
-                                                     float* __restrict__ part) {
+                                                     TP* __restrict__ part, int sym = 0) {
+    constexpr bool F64 = sizeof(TP) == 8;
+    if (sym && blockIdx.y > blockIdx.z) return;
+    const bool same = sym && blockIdx.y == blockIdx.z;
     const int lane = threadIdx.x;
     const int li = lane & 15, lq = lane >> 4;
     const int i0 = blockIdx.y * 64, j0 = blockIdx.z * 64;
     const int64_t kbeg = (int64_t)blockIdx.x * kc;
     const int64_t kend = kbeg + kc < m ? kbeg + kc : m;
     f4v acc[4][4];
+    double acc64[F64 ? 4 : 1][F64 ? 4 : 1][F64 ? 4 : 1];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f4v{0, 0, 0, 0};
+    if constexpr (F64) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc64[a][b][v] = 0.0;
+    }
     // the next 32-row step's loads are issued before this step's splits and
     // MFMAs (few waves per SIMD: each wave hides its own latency)
     float vq[4][8], vx[4][8];
     if (kbeg < kend) {
 #pragma unroll
         for (int a = 0; a < 4; ++a) load_eight<VEC>(vq[a], Q, ldq, kbeg + 8 * lq, i0 + 16 * a + li, kend, n1);
+        if (!same) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) load_eight<VEC>(vx[b], A, lda, kbeg + 8 * lq, j0 + 16 * b + li, kend, n2);
+            for (int b = 0; b < 4; ++b) load_eight<VEC>(vx[b], A, lda, kbeg + 8 * lq, j0 + 16 * b + li, kend, n2);
+        }
     }
 #pragma unroll 1
     for (int64_t r0 = kbeg; r0 < kend; r0 += 32) {
@@ -322,20 +354,30 @@ __global__ __launch_bounds__(64) void gemm_tn_bf16x3(int m, int n1, int n2, cons
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             sq[a] = split3(vq[a]);
-            sx[a] = split3(vx[a]);
+            sx[a] = same ? sq[a] : split3(vx[a]);
         }
         if (more) {
 #pragma unroll
             for (int a = 0; a < 4; ++a) load_eight<VEC>(vq[a], Q, ldq, rn, i0 + 16 * a + li, kend, n1);
+            if (!same) {
 #pragma unroll
-            for (int b = 0; b < 4; ++b) load_eight<VEC>(vx[b], A, lda, rn, j0 + 16 * b + li, kend, n2);
+                for (int b = 0; b < 4; ++b) load_eight<VEC>(vx[b], A, lda, rn, j0 + 16 * b + li, kend, n2);
+            }
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = mma_bf16x3(sq[a], sx[b], acc[a][b]);
+            for (int b = 0; b < 4; ++b) {
+                if constexpr (F64) {
+                    const f4v step = mma_bf16x3(sq[a], sx[b], f4v{0, 0, 0, 0});
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc64[a][b][v] += (double)step[v];
+                } else {
+                    acc[a][b] = mma_bf16x3(sq[a], sx[b], acc[a][b]);
+                }
+            }
     }
-    float* out = part + (int64_t)blockIdx.x * n1 * n2;
+    TP* out = part + (int64_t)blockIdx.x * n1 * n2;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -343,7 +385,12 @@ __global__ __launch_bounds__(64) void gemm_tn_bf16x3(int m, int n1, int n2, cons
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = i0 + 16 * a + 4 * lq + v, j = j0 + 16 * b + li;
-                if (i < n1 && j < n2) out[i + (int64_t)j * n1] = acc[a][b][v];
+                if (i < n1 && j < n2) {
+                    if constexpr (F64)
+                        out[i + (int64_t)j * n1] = acc64[a][b][v];
+                    else
+                        out[i + (int64_t)j * n1] = acc[a][b][v];
+                }
             }
 }
 
@@ -352,14 +399,14 @@ __global__ __launch_bounds__(64) void gemm_tn_bf16x3(int m, int n1, int n2, cons
 // operand: lane (li, lq) loads rows r0 + 4 li .. +3 of the eight columns
 // k0 + 8 lq + s, s = 0..7, so every load is 16 contiguous bytes.  Its output
 // register v is row r0 + 16 lq + 4 v + t, column j0 + 16 jt + li.
+// The body takes plain pointers, so that X and C may be the same array (the
+// in-place reorthogonalisation, reorth_apply): every load of the wave comes
+// before its stores.
 template <bool VEC>
-__global__ __launch_bounds__(64) void gemm_nn_bf16x3(int m, int K, int N, const float* __restrict__ X, int64_t ldx,
-                                                     const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
-                                                     int64_t ldc) {
+__device__ __forceinline__ void nn_bf16x3_block(int m, int K, int N, const float* X, int64_t ldx, const float* B,
+                                                int64_t ldb, float* C, int64_t ldc, int64_t r0, int j0) {
     const int lane = threadIdx.x;
     const int li = lane & 15, lq = lane >> 4;
-    const int64_t r0 = (int64_t)blockIdx.x * 64;
-    const int j0 = blockIdx.y * 64;
     f4v acc[4][4];  // [t][jt]
 #pragma unroll
     for (int jt = 0; jt < 4; ++jt)
@@ -408,6 +455,152 @@ __global__ __launch_bounds__(64) void gemm_nn_bf16x3(int m, int K, int N, const 
             for (int t = 0; t < 4; ++t) c.v[t] = acc[t][jt][v];
             store_four<float, VEC>(c, C, ldc, r0 + 16 * lq + 4 * v, j0 + 16 * jt + li, m, N);
         }
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(64) void gemm_nn_bf16x3(int m, int K, int N, const float* __restrict__ X, int64_t ldx,
+                                                     const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
+                                                     int64_t ldc) {
+    nn_bf16x3_block<VEC>(m, K, N, X, ldx, B, ldb, C, ldc, (int64_t)blockIdx.x * 64, blockIdx.y * 64);
+}
+
+// ---------------------------------------------------------------------------
+// Whole-matrix reorthogonalisation of the mixed-precision recursive QR (one
+// CholeskyQR pass over the finished Q, n <= kReorthMax columns; no
+// counterpart in the reference, whose qr couples once, block_QR.h:88-93).
+// The coupled column blocks of the recursion lose orthogonality to each
+// other in proportion to cond(R22) times the accuracy of the coupling
+// products; one CholeskyQR pass over the nearly orthonormal Q restores it
+// independently of cond(A):
+//   G = Q^T Q (BF16x3 products, each 32-row step added to FP64 sums),
+//   e = ||G - I||_F;
+//   if e > tol: R_c = chol(G), Q <- Q R_c^{-1} = Q - Q Dneg with
+//   Dneg = I - R_c^{-1}, and R <- R_c R.
+// Dneg is small, so holding it in FP32 keeps Q's own bits in the sum
+// (rounding R_c^{-1} itself to FP32 would perturb every column's norm by
+// u32), and its BF16x3 products on the matrix cores are accurate far beyond
+// what the sum keeps.  Kernels: reorth_chol (one workgroup: e, the
+// decision, R_c in FP64), reorth_inv (one workgroup per column of Dneg),
+// reorth_rupdate (one workgroup per column of R), reorth_apply (one wave per
+// 64 rows of Q and 64 columns).  Each does nothing unless reorth_chol set
+// st[0].
+constexpr int kReorthMax = 128;
+constexpr int kReorthThreads = 256;
+
+// G: n x n FP64 (ld n).  On e > tol and a successful Cholesky: Rc (n x n
+// FP64, ld n, zeros below the diagonal) = chol(G) and st[0] = 1; otherwise
+// st[0] = 0 (st[1] = 2 if the Cholesky broke down).  *e_out = e.
+template <int UNUSED = 0>
+__global__ __launch_bounds__(kReorthThreads) void reorth_chol(int n, const double* __restrict__ G,
+                                                             double* __restrict__ Rc, int* __restrict__ st,
+                                                             double* __restrict__ e_out, double tol) {
+    __shared__ double L[kReorthMax][kReorthMax + 1];
+    __shared__ double red[kReorthThreads];
+    __shared__ int ok;
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int e = t; e < n * n; e += kReorthThreads) {
+        const int i = e % n, j = e / n;
+        const double g = G[e];
+        L[i][j] = g;
+        const double d = g - (i == j ? 1.0 : 0.0);
+        s += d * d;
+    }
+    red[t] = s;
+    __syncthreads();
+    for (int w = kReorthThreads / 2; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    const double err = __builtin_sqrt(red[0]);
+    if (t == 0) {
+        *e_out = err;
+        st[0] = 0;
+        st[1] = 0;
+        ok = 1;
+    }
+    if (!(err > tol)) return;  // orthonormal enough (or NaN): Q and R are kept
+    __syncthreads();
+    // right-looking Cholesky, upper factor in place: row j of R_c, then the
+    // trailing upper triangle
+    for (int j = 0; j < n; ++j) {
+        if (t == 0) {
+            const double p = L[j][j];
+            if (!(p > 0.0) || !(p < __builtin_inf())) ok = 0;
+            L[j][j] = ok ? __builtin_sqrt(p) : 1.0;
+        }
+        __syncthreads();
+        const double dj = L[j][j];
+        for (int c = j + 1 + t; c < n; c += kReorthThreads) L[j][c] /= dj;
+        __syncthreads();
+        // trailing update over a 16 x 16 grid of threads (no index division)
+        for (int r = j + 1 + (t >> 4); r < n; r += 16) {
+            const double ljr = L[j][r];
+            for (int c = r + ((j + 1 + (t & 15) - r) & 15); c < n; c += 16) L[r][c] -= ljr * L[j][c];
+        }
+        __syncthreads();
+    }
+    if (!ok) {
+        if (t == 0) st[1] = 2;
+        return;
+    }
+    for (int e = t; e < n * n; e += kReorthThreads) {
+        const int i = e % n, j = e / n;
+        Rc[e] = i <= j ? L[i][j] : 0.0;
+    }
+    if (t == 0) st[0] = 1;
+}
+
+// Column c = blockIdx.x of Dneg = I - R_c^{-1} (FP32, n x n, ld n): R_c x = e_c
+// by column-oriented back substitution (x_i /= R_ii, then x_k -= R_ki x_i
+// for k < i, in parallel over k), in FP64.
+template <int UNUSED = 0>
+__global__ __launch_bounds__(kReorthMax) void reorth_inv(int n, const double* __restrict__ Rc, float* __restrict__ D,
+                                                         const int* __restrict__ st) {
+    __shared__ double x[kReorthMax];
+    if (!st[0]) return;
+    const int c = blockIdx.x, t = threadIdx.x;
+    if (t < n) x[t] = t == c ? 1.0 : 0.0;
+    __syncthreads();
+    for (int i = c; i >= 0; --i) {
+        if (t == 0) x[i] /= Rc[i + (int64_t)i * n];
+        __syncthreads();
+        const double xi = x[i];
+        if (t < i) x[t] -= Rc[t + (int64_t)i * n] * xi;
+        __syncthreads();
+    }
+    if (t < n) D[t + (int64_t)c * n] = t <= c ? (float)((t == c ? 1.0 : 0.0) - x[t]) : 0.f;
+}
+
+// Column j = blockIdx.x of R <- R_c R (R upper triangular, ld ldr), in FP64.
+template <typename T>
+__global__ __launch_bounds__(kReorthMax) void reorth_rupdate(int n, const double* __restrict__ Rc, T* __restrict__ R,
+                                                             int64_t ldr, const int* __restrict__ st) {
+    __shared__ double col[kReorthMax];
+    if (!st[0]) return;
+    const int j = blockIdx.x, t = threadIdx.x;
+    if (t < n) col[t] = t <= j ? (double)R[t + (int64_t)j * ldr] : 0.0;
+    __syncthreads();
+    if (t <= j && t < n) {
+        double acc = 0.0;
+        for (int k = t; k <= j; ++k) acc += Rc[t + (int64_t)k * n] * col[k];
+        R[t + (int64_t)j * ldr] = (T)acc;
+    }
+}
+
+// Columns [c0, c0 + N) of Q (m x *, ld ldq) += Q[:, 0:K] D[0:K, c0:c0 + N] in
+// place, if st[0] is set, one wave per 64 rows: the BF16x3 product of
+// gemm_nn_bf16x3 with B = Dneg = I - R_c^{-1} (C -= X Dneg).  D is upper
+// triangular, so output columns [c0, c0 + N) read input columns < c0 + N
+// only: the column groups are launched from the last to the first, and no
+// launch reads a column an earlier one wrote.
+template <bool VEC>
+__global__ __launch_bounds__(64) void reorth_apply(int m, int K, int N, float* Q, int64_t ldq, int c0,
+                                                   const float* __restrict__ Dneg, int64_t ldd,
+                                                   const int* __restrict__ st) {
+    if (!st[0]) return;
+    nn_bf16x3_block<VEC>(m, K, N, Q, ldq, Dneg + (int64_t)c0 * ldd, ldd, Q + (int64_t)c0 * ldq, ldq,
+                         (int64_t)blockIdx.x * 64, 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -258,17 +258,91 @@ size_t panel_ws_bytes(int m, int n) {
     return a > b ? a : b;
 }
 
-// Panel TSQR workspace, then the split-K partials, then n1 x n2 elements for
-// the mixed method's second coupling product.
+// Workspace of the recursive QR (byte offsets, 256-byte aligned): the panel
+// TSQR workspace, the split-K partials (of the coupling products, and of the
+// mixed method's final Gram matrix Q^T Q, FP64, for 32 < n <= kReorthMax),
+// n1 x n2 elements for the mixed method's second coupling product, then the
+// final reorthogonalisation's G (n x n FP64), D (n x n FP32) and its status
+// words and residual.
+struct QrLayout {
+    size_t panel, parts, tmp, gram, dmat, stat, total;
+};
+
+// Rows per split-K chunk of the final Gram matrix Q^T Q (n x n).
+inline int reorth_chunk_rows(int m, int n) { return tn_chunk_rows(m, n, n); }
+
+template <typename T>
+QrLayout qr_layout(int m, int n) {
+    QrLayout L{};
+    L.panel = 0;
+    const size_t panel = panel_ws_bytes<T>(m, n);
+    if (panel == 0) return L;
+    // FP64 partials for the mixed method's coupling products (FP32 data)
+    size_t parts = tn_partial_elems(m, n) * sizeof(double);
+    const bool reorth = n > 32 && n <= kReorthMax;
+    if (reorth) {
+        const int kc = reorth_chunk_rows(m, n);
+        const size_t g = (size_t)((m + kc - 1) / kc) * n * n * sizeof(double);
+        parts = parts > g ? parts : g;
+    }
+    const int n1 = split_cols(n);
+    L.parts = align256(panel);
+    L.tmp = L.parts + align256(parts);
+    L.gram = L.tmp + align256((size_t)n1 * (n - n1) * sizeof(T));
+    L.dmat = L.gram + (reorth ? align256((size_t)n * n * sizeof(double)) : 0);
+    L.stat = L.dmat + (reorth ? align256((size_t)n * n * sizeof(float)) : 0);
+    L.total = L.stat + (reorth ? 256 : 0);
+    return L;
+}
+
 template <typename T>
 size_t qr_ws_bytes(int m, int n) {
     if (n < 1 || m < n) return 0;
     if (n <= 32) return tsqr_ws_bytes<T>(m, n);
-    const size_t panel = panel_ws_bytes<T>(m, n);
-    if (panel == 0) return 0;
-    const int n1 = split_cols(n);
-    return align256(panel) + align256(tn_partial_elems(m, n) * sizeof(T)) +
-           align256((size_t)n1 * (n - n1) * sizeof(T));
+    return qr_layout<T>(m, n).total;
+}
+
+// Final CholeskyQR pass of the mixed-precision recursive QR over the whole
+// Q (32 < n <= kReorthMax): G = Q^T Q from the BF16x3 split-K products with
+// each 32-row step added to FP64 sums (tiles on and above the diagonal),
+// e = ||G - I||_F; if
+// e > tol, R_c = chol(G), Q <- Q R_c^{-1} in place and R <- R_c R
+// (gemm_mfma.hpp).  R_c overwrites G.
+int reorth(int m, int n, float* Q, int ldq, float* R, int ldr, char* ws, const QrLayout& L, double tol,
+           hipStream_t st) {
+    double* part = reinterpret_cast<double*>(ws + L.parts);
+    double* G = reinterpret_cast<double*>(ws + L.gram);
+    float* D = reinterpret_cast<float*>(ws + L.dmat);
+    int* stat = reinterpret_cast<int*>(ws + L.stat);
+    double* resid = reinterpret_cast<double*>(ws + L.stat + 16);
+    const int kc = reorth_chunk_rows(m, n);
+    const int chunks = (m + kc - 1) / kc;
+    const bool vec = aligned16(Q, ldq, sizeof(float));
+    const dim3 gtn(chunks, (n + 63) / 64, (n + 63) / 64);
+    if (vec)
+        hipLaunchKernelGGL((gemm_tn_bf16x3<true, double>), gtn, dim3(64), 0, st, m, n, n, (const float*)Q,
+                           (int64_t)ldq, (const float*)Q, (int64_t)ldq, kc, part, 1);
+    else
+        hipLaunchKernelGGL((gemm_tn_bf16x3<false, double>), gtn, dim3(64), 0, st, m, n, n, (const float*)Q,
+                           (int64_t)ldq, (const float*)Q, (int64_t)ldq, kc, part, 1);
+    hipLaunchKernelGGL((gemm_tn_reduce<double, double>), dim3((n * n + 15) / 16), dim3(256), 0, st, n, n, chunks,
+                       (const double*)part, G, (int64_t)n, 0, 1);
+    hipLaunchKernelGGL(reorth_chol<>, dim3(1), dim3(kReorthThreads), 0, st, n, (const double*)G, G, stat, resid, tol);
+    hipLaunchKernelGGL(reorth_inv<>, dim3(n), dim3(kReorthMax), 0, st, n, (const double*)G, D, (const int*)stat);
+    hipLaunchKernelGGL(reorth_rupdate<float>, dim3(n), dim3(kReorthMax), 0, st, n, (const double*)G, R, (int64_t)ldr,
+                       (const int*)stat);
+    // Q <- Q - Q Dneg, the last column group first (reorth_apply)
+    const dim3 gap((m + 63) / 64);
+    for (int c0 = ((n - 1) / 64) * 64; c0 >= 0; c0 -= 64) {
+        const int N = n - c0 < 64 ? n - c0 : 64;
+        if (vec)
+            hipLaunchKernelGGL(reorth_apply<true>, gap, dim3(64), 0, st, m, c0 + N, N, Q, (int64_t)ldq, c0,
+                               (const float*)D, (int64_t)n, (const int*)stat);
+        else
+            hipLaunchKernelGGL(reorth_apply<false>, gap, dim3(64), 0, st, m, c0 + N, N, Q, (int64_t)ldq, c0,
+                               (const float*)D, (int64_t)n, (const int*)stat);
+    }
+    return hipGetLastError() == hipSuccess ? TSQR_OK : TSQR_ERR_HIP;
 }
 
 // One coupling step: part = chunk products of Q1^T A2, C = their sum (or
@@ -287,14 +361,18 @@ int couple(int m, int n1, int n2, const T* A1, T* A2, int lda, T* C, int ldc, T*
         if (bf16) {
             const dim3 gtn(chunks, (n1 + 63) / 64, (n2 + 63) / 64), gnn((m + 63) / 64, (n2 + 63) / 64);
             if (timed) timing_mark(kPhaseGemmTN, true, st);
+            // each 32-row step summed into FP64 (part holds FP64 partials):
+            // the error of R12 sets how far the coupled blocks stay from
+            // orthogonal, amplified by cond(R22)
+            double* part64 = reinterpret_cast<double*>(part);
             if (vec)
-                hipLaunchKernelGGL(gemm_tn_bf16x3<true>, gtn, dim3(64), 0, st, m, n1, n2, A1, (int64_t)lda,
-                                   (const T*)A2, (int64_t)lda, kc, part);
+                hipLaunchKernelGGL((gemm_tn_bf16x3<true, double>), gtn, dim3(64), 0, st, m, n1, n2, A1, (int64_t)lda,
+                                   (const T*)A2, (int64_t)lda, kc, part64);
             else
-                hipLaunchKernelGGL(gemm_tn_bf16x3<false>, gtn, dim3(64), 0, st, m, n1, n2, A1, (int64_t)lda,
-                                   (const T*)A2, (int64_t)lda, kc, part);
-            hipLaunchKernelGGL(gemm_tn_reduce<T>, dim3((n1 * n2 + 15) / 16), dim3(256), 0, st, n1, n2, chunks,
-                               (const T*)part, C, (int64_t)ldc, 0);
+                hipLaunchKernelGGL((gemm_tn_bf16x3<false, double>), gtn, dim3(64), 0, st, m, n1, n2, A1,
+                                   (int64_t)lda, (const T*)A2, (int64_t)lda, kc, part64);
+            hipLaunchKernelGGL((gemm_tn_reduce<double, T>), dim3((n1 * n2 + 15) / 16), dim3(256), 0, st, n1, n2,
+                               chunks, (const double*)part64, C, (int64_t)ldc, 0);
             if (timed) timing_mark(kPhaseGemmTN, false, st);
             if (timed) timing_mark(kPhaseGemmNN, true, st);
             if (vec)
@@ -368,15 +446,19 @@ int qr_stream(int m, int n, T* A, int lda, T* R, int ldr, void* work, size_t wor
     if (need == 0) return TSQR_ERR_SHAPE;
     if (work_bytes < need || !work) return TSQR_ERR_WORKSPACE;
     if (n <= 32) return tsqr_core<T>(m, n, A, lda, R, ldr, work, st, mixed);
-    const size_t panel = align256(panel_ws_bytes<T>(m, n));
-    const size_t parts = align256(tn_partial_elems(m, n) * sizeof(T));
+    const QrLayout L = qr_layout<T>(m, n);
     char* ws = reinterpret_cast<char*>(work);
     // every R21 block of the recursion at once (the panels write their own
     // diagonal blocks, zeros included)
     hipLaunchKernelGGL(zero_lower<T>, dim3((unsigned)(((int64_t)n * n + 255) / 256)), dim3(256), 0, st, R,
                        (int64_t)ldr, n);
-    return qr_rec<T>(m, n, A, lda, R, ldr, ws, panel, reinterpret_cast<T*>(ws + panel),
-                     reinterpret_cast<T*>(ws + panel + parts), st, mixed);
+    e = qr_rec<T>(m, n, A, lda, R, ldr, ws, L.parts, reinterpret_cast<T*>(ws + L.parts),
+                  reinterpret_cast<T*>(ws + L.tmp), st, mixed);
+    if (e) return e;
+    if constexpr (sizeof(T) == 4) {
+        if (mixed && n <= kReorthMax) return reorth(m, n, A, lda, R, ldr, ws, L, mixed_opts().tol, st);
+    }
+    return TSQR_OK;
 }
 
 // Eigenvalues of the symmetric n x n matrix S (row-major, ld 32, n <= 32;
