@@ -11,8 +11,12 @@ factors the Q left by step k-1 with exactly the same work; A is not
 re-uploaded between steps.
 
 metric  TSQR HBM GB/s = B_alg / (seconds per step), with
-        B_alg = (3mn + n^2) * 4 bytes (A read twice and written once are the
-        lower bound for an explicit Q that does not fit on chip).
+        B_alg = (3mn + n^2) * s bytes when A does not fit in the aggregate
+        LDS of the part (256 CUs x 160 KiB = 40 MiB): A read twice and
+        written once are then the lower bound for an explicit Q; and
+        B_alg = (2mn + n^2) * s (read A once, write Q and R once) when it
+        does, as for the 27648 x 32 and 27648 x 128 configurations.  The
+        compulsory (2mn + n^2) * s is reported beside it.
 roofline  for the dominant phase of the two streaming passes (leaf R pass:
         reads A, m*n*4 algorithmic bytes; leaf Q pass: reads A and writes Q,
         2*m*n*4), from HIP events on the launch stream over a second pass of
@@ -20,7 +24,9 @@ roofline  for the dominant phase of the two streaming passes (leaf R pass:
         a few microseconds of GPU time), against the 8 TB/s HBM peak.
 cpu_baseline  the oracle's C restatement of the reference algorithm
         (OpenMP over tiles, all host cores) on a bounded sample of the same
-        workload (m/8 rows), plus OpenBLAS sgeqrf+sorgqr on the same sample.
+        workload: the first m/8 rows of the very array the GPU factors,
+        copied to the host before timing; plus OpenBLAS sgeqrf+sorgqr on the
+        same sample; with the host CPU model and thread count.
 
 batched  (same line, key "batched") the headline's second half: QRs/s of the
         512-matrix uneven batch, mixed method vs plain FP32 Householder, each
@@ -45,6 +51,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Aggregate LDS of the part (256 CUs x 160 KiB): below it A could stay on chip
+# between the R and Q passes, so only (2mn + n^2) * s bytes are compulsory.
+AGGREGATE_LDS_BYTES = 256 * 160 * 1024
 FP64_MFMA_PEAK_TFLOPS = 78.6
 BF16_MFMA_PEAK_TFLOPS = 2500.0
 # Measured HBM bytes per dispatch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
@@ -74,6 +83,24 @@ WORKLOADS = {
     "tsqr_mixed_max_fp32": ("tsqr_mixed", 23887872, 32, "float32"),
     "tsqr_mixed_27648x32_fp32": ("tsqr_mixed", 27648, 32, "float32"),
 }
+
+
+def algorithmic_bytes(m: int, n: int, esz: int) -> int:
+    """B_alg of one explicit-Q factorisation (see the module docstring)."""
+    if m * n * esz <= AGGREGATE_LDS_BYTES:
+        return (2 * m * n + n * n) * esz
+    return (3 * m * n + n * n) * esz
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def parse():
@@ -222,28 +249,38 @@ def batched_leg(steps: int, count: int) -> dict:
     return out
 
 
-def cpu_baseline(m: int, n: int, dtype: str, frac: int) -> dict:
-    """Times the oracle restatement and LAPACK on a bounded host sample."""
-    import numpy as np
+def cpu_sample_rows(m: int, n: int, frac: int) -> int:
+    """Rows of the bounded CPU sample: m / frac, rounded down to whole
+    27648-row chunks of the reference (NUM_SM * BLOCK_SIZE), all of m when
+    m is small."""
+    return max(n, (m // frac) // 27648 * 27648) if m >= 27648 * frac else m
 
+
+def cpu_baseline(A: np.ndarray, frac: int) -> dict:
+    """Times the oracle restatement and LAPACK on a bounded host sample A
+    (Fortran-ordered ms x n, the first rows of the GPU's input)."""
     from oracle import oracle as O  # the cpu_baseline leg is the only bench user of the oracle
 
-    ms = max(n, (m // frac) // 27648 * 27648) if m >= 27648 * frac else m
-    A = O.gen_uniform(ms, n, np.dtype(dtype))
+    ms, n = A.shape
+    dtype = A.dtype.name
     cores = os.cpu_count() or 1
-    esz = np.dtype(dtype).itemsize
-    balg = (3 * ms * n + n * n) * esz
+    esz = A.dtype.itemsize
+    balg = algorithmic_bytes(ms, n, esz)
     t0 = time.perf_counter()
-    O.tsqr_ref(A, bs=256 if dtype == "float32" else 128, num_sm=108)
+    ref = O.tsqr_ref if n <= 32 else O.qr_ref  # tsqr<T> / qr<T> of the reference
+    ref(A, bs=256 if dtype == "float32" else 128, num_sm=108)
     t_port = time.perf_counter() - t0
     out = {
         "value": round(balg / t_port / 1e9, 3),
         "unit": "GB/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{ms}x{n} {dtype} (1/{frac} of the rows), oracle restatement of the reference "
-                  f"tsqr (NUM_SM=108, BLOCK_SIZE={256 if dtype == 'float32' else 128}), OpenMP over tiles",
+        "sample": f"the first {ms} rows x {n} {dtype} of the GPU's input (1/{frac} of the rows, copied from the "
+                  f"device), oracle restatement of the reference {'tsqr' if n <= 32 else 'qr'} (NUM_SM=108, "
+                  f"BLOCK_SIZE={256 if dtype == 'float32' else 128}), OpenMP over tiles",
         "seconds": round(t_port, 3),
+        "cpu": cpu_model(),
+        "algorithmic_bytes": balg,
     }
     try:
         import scipy.linalg.lapack as la
@@ -316,6 +353,11 @@ def main():
     g.manual_seed(0)
     A = torch.empty((n, m), dtype=tdt, device=dev)
     A.uniform_(-1.0, 1.0, generator=g)  # column-major m x n with lda = m
+    # the CPU baseline's sample: the first rows of this very array
+    host_sample = None
+    if not args.no_cpu:
+        ms = cpu_sample_rows(m, n, args.cpu_fraction)
+        host_sample = A[:, :ms].cpu().numpy().T  # Fortran-ordered ms x n
     R = torch.zeros((n, n), dtype=tdt, device=dev)
     L = tsqr_amd.lib()
     need = (L.qr_workspace_bytes if kind == "qr" else L.tsqr_workspace_bytes)(m, n, code)
@@ -374,12 +416,12 @@ def main():
     print(f"# phases (ms total, count): {phases}", file=sys.stderr, flush=True)
 
     sec = (t1 - t0) / args.steps
-    balg = (3 * m * n + n * n) * esz if is_tsqr else None
-    if kind == "qr":
-        # n = 32*2^k: every column block is read twice and written once by its
-        # panel TSQR, plus the coupling products' traffic; report the panel rate
-        balg = (3 * m * n + n * n) * esz
+    # the recursive qr is counted like a TSQR of the whole m x n (the
+    # coupling products' extra traffic is not credited)
+    balg = algorithmic_bytes(m, n, esz)
+    compulsory = (2 * m * n + n * n) * esz
     value = balg / sec / 1e9
+    print(f"{kind} Latency: {sec * 1e3:f} ms", file=sys.stderr, flush=True)  # test_tsqr.cu:90
 
     # Algorithmic bytes per launch of the two streaming kernels: the leaf R
     # pass reads A (m*n elements), the leaf Q pass reads A and writes Q.
@@ -480,6 +522,8 @@ def main():
             "m": m,
             "n": n,
             "algorithmic_bytes_per_step": balg,
+            "compulsory_bytes_per_step": compulsory,
+            "compulsory_GBs": round(compulsory / sec / 1e9, 2),
             "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
             "phase_ms": phase_ms,
             "device_copy_GBs": copy_gbs,
@@ -493,7 +537,7 @@ def main():
         out["batched"] = batched_leg(args.batch_steps, args.batch_count)
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(m, n, dtype, args.cpu_fraction)
+        out["cpu_baseline"] = cpu_baseline(host_sample, args.cpu_fraction)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -711,50 +711,50 @@ constexpr int stream_slabs() {
     return sizeof(T) == 4 ? 4 : 2;
 }
 
-// Rows [r0, r0 + 4) of column c, all inside the matrix (no row check); zero
-// for the padding columns c >= ncols.
+// Rows [r0, r0 + 4) of column c, all inside the matrix and c a real column
+// (no checks).  The streaming kernel loads the padding columns c >= n as
+// column 0 and drops them when it converts (gram_chunk's in0 / in1): a
+// select right after the load, or a branch around it, made the compiler wait
+// for every load in flight, prefetches included, before the next
+// instruction.
 template <typename T, bool VEC>
-__device__ __forceinline__ Raw4<T> load_full4(const T* __restrict__ base, int64_t ld, int r0, int c, int ncols) {
+__device__ __forceinline__ Raw4<T> load_col4(const T* __restrict__ base, int64_t ld, int r0, int c) {
     Raw4<T> o;
-    if (c < ncols) {
-        const T* p = base + (int64_t)c * ld + r0;
-        if constexpr (VEC && sizeof(T) == 4) {
-            typedef float f4 __attribute__((ext_vector_type(4)));
-            const f4 q = *reinterpret_cast<const f4*>(p);
+    const T* p = base + (int64_t)c * ld + r0;
+    if constexpr (VEC && sizeof(T) == 4) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const f4 q = *reinterpret_cast<const f4*>(p);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) o.v[t] = q[t];
-        } else if constexpr (VEC) {
-            typedef double d2 __attribute__((ext_vector_type(2)));
-            const d2 q0 = *reinterpret_cast<const d2*>(p);
-            const d2 q1 = *reinterpret_cast<const d2*>(p + 2);
-            o.v[0] = q0[0];
-            o.v[1] = q0[1];
-            o.v[2] = q1[0];
-            o.v[3] = q1[1];
-        } else {
-#pragma unroll
-            for (int t = 0; t < 4; ++t) o.v[t] = p[t];
-        }
+        for (int t = 0; t < 4; ++t) o.v[t] = q[t];
+    } else if constexpr (VEC) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const d2 q0 = *reinterpret_cast<const d2*>(p);
+        const d2 q1 = *reinterpret_cast<const d2*>(p + 2);
+        o.v[0] = q0[0];
+        o.v[1] = q0[1];
+        o.v[2] = q1[0];
+        o.v[3] = q1[1];
     } else {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) o.v[t] = T(0);
+        for (int t = 0; t < 4; ++t) o.v[t] = p[t];
     }
     return o;
 }
 
 // FP64 MFMA Gram update of one chunk held as S slabs of 16 rows (lane l:
-// rows 16q + 4(l/16) .. +3 of columns l%16 and 16 + l%16).
+// rows 16q + 4(l/16) .. +3 of columns l%16 and 16 + l%16); values of a
+// lane whose column is padding (in0 / in1 false) count as zero.
 template <typename T, int NT, int S>
 __device__ __forceinline__ void gram_chunk(const Raw4<T> (&c0)[S], const Raw4<T> (&c1)[S], v4d& a00, v4d& a01,
-                                           v4d& a11) {
+                                           v4d& a11, bool in0 = true, bool in1 = true) {
 #pragma unroll
     for (int q = 0; q < S; ++q) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const double x0 = (double)c0[q].v[t];
+            const double x0 = in0 ? (double)c0[q].v[t] : 0.0;
             a00 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, x0, a00, 0, 0, 0);
             if constexpr (NT == 2) {
-                const double x1 = (double)c1[q].v[t];
+                const double x1 = in1 ? (double)c1[q].v[t] : 0.0;
                 a01 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, x1, a01, 0, 0, 0);
                 a11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, x1, a11, 0, 0, 0);
             }
@@ -865,34 +865,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
     constexpr int CR = 16 * S;  // rows per chunk
     const int nfull = m / CR;   // complete chunks
     v4d a00 = {0, 0, 0, 0}, a01 = {0, 0, 0, 0}, a11 = {0, 0, 0, 0};
-    if (w < nfull) {
-        Raw4<T> c0[S], c1[S];
+    // padding columns (n < 32) load column 0 and are dropped in gram_chunk
+    const bool in0 = col < n, in1 = col + 16 < n;
+    const int c0col = in0 ? col : 0, c1col = in1 ? col + 16 : 0;
+    // Two chunk buffers in turn (ping-pong): the loads of the next chunk are
+    // issued before the MFMAs of the current one and consumed after them.
+    // The scheduling barriers keep that order; without them the compiler
+    // folded the two buffers into one and each wave waited for its loads
+    // right before using them.  The last step reloads a chunk it already
+    // has instead of branching around the loads.
+    auto load_chunk = [&](int ch, Raw4<T>(&x0)[S], Raw4<T>(&x1)[S]) {
 #pragma unroll
         for (int q = 0; q < S; ++q) {
-            c0[q] = load_full4<T, VEC>(A, lda, CR * w + 16 * q + rq, col, n);
-            if constexpr (NT == 2) c1[q] = load_full4<T, VEC>(A, lda, CR * w + 16 * q + rq, col + 16, n);
+            x0[q] = load_col4<T, VEC>(A, lda, CR * ch + 16 * q + rq, c0col);
+            if constexpr (NT == 2) x1[q] = load_col4<T, VEC>(A, lda, CR * ch + 16 * q + rq, c1col);
         }
+    };
+    if (w < nfull) {
+        // this wave's chunks are w + k W, k = 0 .. nch - 1, taken in pairs
+        // with no exit inside the pair, so the compiler keeps every load
+        // where it is written
+        const int nch = (nfull - w + W - 1) / W;
+        Raw4<T> a0[S], a1[S], b0[S], b1[S];
+        load_chunk(w, a0, a1);
 #pragma unroll 1
-        for (int ch = w;;) {
-            const int next = ch + W;
-            const bool more = next < nfull;
-            Raw4<T> p0[S], p1[S];
-            if (more) {
-#pragma unroll
-                for (int q = 0; q < S; ++q) {
-                    p0[q] = load_full4<T, VEC>(A, lda, CR * next + 16 * q + rq, col, n);
-                    if constexpr (NT == 2) p1[q] = load_full4<T, VEC>(A, lda, CR * next + 16 * q + rq, col + 16, n);
-                }
-            }
-            gram_chunk<T, NT, S>(c0, c1, a00, a01, a11);
-            if (!more) break;
-            ch = next;
-#pragma unroll
-            for (int q = 0; q < S; ++q) {
-                c0[q] = p0[q];
-                if constexpr (NT == 2) c1[q] = p1[q];
-            }
+        for (int k = 0; k + 1 < nch; k += 2) {
+            load_chunk(w + (k + 1) * W, b0, b1);
+            __builtin_amdgcn_sched_barrier(0);
+            gram_chunk<T, NT, S>(a0, a1, a00, a01, a11, in0, in1);
+            __builtin_amdgcn_sched_barrier(0);
+            const int ka = k + 2 < nch ? k + 2 : nch - 1;  // the last pair reloads a chunk it has
+            load_chunk(w + ka * W, a0, a1);
+            __builtin_amdgcn_sched_barrier(0);
+            gram_chunk<T, NT, S>(b0, b1, a00, a01, a11, in0, in1);
+            __builtin_amdgcn_sched_barrier(0);
         }
+        if (nch & 1) gram_chunk<T, NT, S>(a0, a1, a00, a01, a11, in0, in1);
     }
     if (m % CR && w == nfull % W) {  // the short last chunk, bounds-checked loads
         Raw4<T> c0[S], c1[S];

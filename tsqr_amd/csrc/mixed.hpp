@@ -186,38 +186,68 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
     GramAcc32 g;
     g.sums = sums[wv];
     g.zero(lane);
-    if (w < chunks) {
+    // Full 64-row chunks in two buffers in turn (ping-pong, as
+    // tsqr_gram_stream): unconditional loads issued before the previous
+    // chunk's products, padding columns (n < 32) loaded as column 0 and
+    // zeroed only where they are used.
+    const bool in0 = col < n, in1 = col + 16 < n;
+    const int c0col = in0 ? col : 0, c1col = in1 ? col + 16 : 0;
+    const int nfull = m / 64;
+    auto load_chunk = [&](int ch, float(&x0)[D][8], float(&x1)[D][8]) {
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const Raw4<float> a = load_col4<float, VEC>(A, lda, 64 * ch + 32 * q + rq + 4 * h, c0col);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) x0[q][4 * h + t] = a.v[t];
+                if constexpr (NT == 2) {
+                    const Raw4<float> b = load_col4<float, VEC>(A, lda, 64 * ch + 32 * q + rq + 4 * h, c1col);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) x1[q][4 * h + t] = b.v[t];
+                }
+            }
+        }
+    };
+    auto use_chunk = [&](float(&x0)[D][8], float(&x1)[D][8]) {
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                x0[q][k] = in0 ? x0[q][k] : 0.f;
+                if constexpr (NT == 2) x1[q][k] = in1 ? x1[q][k] : 0.f;
+            }
+            g.add<NT>(x0[q], x1[q]);
+        }
+        g.flush<NT>(lane);
+    };
+    if (w < nfull) {
+        const int nch = (nfull - w + W - 1) / W;
+        float a0[D][8], a1[D][8], b0[D][8], b1[D][8];
+        load_chunk(w, a0, a1);
+#pragma unroll 1
+        for (int k = 0; k + 1 < nch; k += 2) {
+            load_chunk(w + (k + 1) * W, b0, b1);
+            __builtin_amdgcn_sched_barrier(0);
+            use_chunk(a0, a1);
+            __builtin_amdgcn_sched_barrier(0);
+            const int ka = k + 2 < nch ? k + 2 : nch - 1;  // the last pair reloads a chunk it has
+            load_chunk(w + ka * W, a0, a1);
+            __builtin_amdgcn_sched_barrier(0);
+            use_chunk(b0, b1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (nch & 1) use_chunk(a0, a1);
+    }
+    if (chunks > nfull && w == nfull % W) {  // the short last chunk, bounds-checked loads
         float c0[D][8], c1[D][8];
 #pragma unroll
         for (int q = 0; q < D; ++q) {
-            load8<VEC>(c0[q], A, lda, 64 * w + 32 * q + rq, col, m, n);
-            if constexpr (NT == 2) load8<VEC>(c1[q], A, lda, 64 * w + 32 * q + rq, col + 16, m, n);
+            load8<VEC>(c0[q], A, lda, 64 * nfull + 32 * q + rq, col, m, n);
+            if constexpr (NT == 2) load8<VEC>(c1[q], A, lda, 64 * nfull + 32 * q + rq, col + 16, m, n);
+            g.add<NT>(c0[q], c1[q]);
         }
-#pragma unroll 1
-        for (int ch = w;;) {
-            const int next = ch + W;
-            const bool more = next < chunks;
-            float p0[D][8], p1[D][8];
-            if (more) {
-#pragma unroll
-                for (int q = 0; q < D; ++q) {
-                    load8<VEC>(p0[q], A, lda, 64 * next + 32 * q + rq, col, m, n);
-                    if constexpr (NT == 2) load8<VEC>(p1[q], A, lda, 64 * next + 32 * q + rq, col + 16, m, n);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < D; ++q) g.add<NT>(c0[q], c1[q]);
-            g.flush<NT>(lane);
-            if (!more) break;
-            ch = next;
-#pragma unroll
-            for (int q = 0; q < D; ++q)
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    c0[q][k] = p0[q][k];
-                    if constexpr (NT == 2) c1[q][k] = p1[q][k];
-                }
-        }
+        g.flush<NT>(lane);
     }
     // the workgroup's sum in L.G, waves added in order (accumulator layout:
     // lane l, register v holds row 4(l/16) + v, column l%16 of a block)
