@@ -193,8 +193,9 @@ def cond_sweep(steps: int, m: int = 1048576) -> dict:
 
 
 def batched_leg(steps: int, count: int) -> dict:
-    """512 uneven FP32 matrices (m log-uniform in [4096, 262144], a multiple
-    of 64; n in {8, 16, 32, 64}; iid N(0,1) from a seeded torch generator),
+    """512 uneven FP32 matrices (tsqr_amd.workloads.batch_shapes: m
+    log-uniform in [4096, 262144] rounded down to a multiple of n, n in
+    {8, 16, 32, 64}; iid N(0,1) from a seeded torch generator),
     factored by one replay of the captured hipGraph per step, mixed method
     and plain (Householder) method.  Each step starts from the same input
     (an untimed device copy restores it); the step is timed with HIP events
@@ -215,9 +216,11 @@ def batched_leg(steps: int, count: int) -> dict:
         a.normal_(generator=g)
     mats = [a.clone(memory_format=torch.preserve_format) for a in src]
     mats = [tsqr_amd.colmajor(a) if a.stride(0) != 1 else a for a in mats]
-    balg = sum((3 * m * n + n * n) * 4 for m, n in shapes)
+    # compulsory bytes per matrix, (2mn + n^2) * 4: read A, write Q and R
+    balg = sum((2 * m * n + n * n) * 4 for m, n in shapes)
     stream = torch.cuda.current_stream(dev)
-    out = {"workload": f"{count} matrices, m log-uniform in [4096, 262144] (multiple of 64), n in {{8,16,32,64}}, "
+    out = {"workload": f"{count} matrices, m log-uniform in [4096, 262144] rounded down to a multiple of n, "
+                       f"n in {{8,16,32,64}}, shapes from std::mt19937_64(0) (tsqr_amd.workloads.batch_shapes), "
                        f"float32 iid N(0,1); one hipGraph replay per batch",
            "matrices": count, "algorithmic_bytes_per_batch": balg}
     for method in ("mixed", "plain"):
