@@ -241,6 +241,23 @@ int tsqr_set_leaf_tiles(int dtype, int n, int log2m, int tiles);
 int tsqr_leaf_tiles(int dtype, int m, int n);
 int tsqr_set_gemm_waves(int waves);
 
+/* More knobs of the same table (no counterpart in the reference, whose tree
+ * fan-in is fixed by BLOCK_SIZE / n, kernelQR.h:3-11, 321-388):
+ *   tsqr_set_tree_arity    children per tree node for this dtype and padded
+ *                          width (2 .. H/n; 0 = H/n, full H-row blocks,
+ *                          the default); fewer children give a deeper tree
+ *                          of smaller nodes;
+ *   tsqr_tree_arity        the arity a call would use now (after clipping);
+ *   tsqr_set_stream_shape  largest grid of the streaming Gram passes (1 ..
+ *                          1024 workgroups of 4 waves; 0 = 256, one per
+ *                          CU, the fastest measured on MI355X) and the
+ *                          waves the Gram Q pass aims for when superleaves
+ *                          are few (0 = 2048).
+ * Process-wide, for calls planned afterwards (workspace sizes included). */
+int tsqr_set_tree_arity(int dtype, int n, int arity);
+int tsqr_tree_arity(int dtype, int m, int n);
+int tsqr_set_stream_shape(int max_groups, int q_waves);
+
 /* After a tsqr_*_stream call that used `work` has been enqueued: waits for
  * the device, counts the level-0 superleaves that took the Gram path and the
  * Householder path, and sets *flat to 1 if the call took the flat Gram path.

@@ -137,12 +137,19 @@ def test_qr_workspace_covers_every_panel_tuned(lib):
     """Same under the measured tuning table, which gives narrower panels
     more, shorter superleaves (262144 x 48 FP64 needed 16.4 MB for the panel
     region and had 19.3 MB worth of writes)."""
-    from tsqr_amd.autotune import DEFAULT_FILE, apply, load, reset
+    from tsqr_amd.autotune import DEFAULT_FILE, apply, load, restore_default
 
     table = load(DEFAULT_FILE)
     try:
         apply(table)
         assert lib.qr_workspace_bytes(262144, 48, 2) >= lib.tsqr_workspace_bytes(262144, 16, 2)
         assert not _qr_covers_panels(lib, MS, NS)
+        # narrowest trees (arity 2, deeper, more blocks) and the smallest
+        # streaming grid
+        for code in (1, 2):
+            for w in (8, 16, 32):
+                assert lib.tsqr_set_tree_arity(code, w, 2) == 0
+        assert lib.tsqr_set_stream_shape(256, 1024) == 0
+        assert not _qr_covers_panels(lib, MS, NS)
     finally:
-        reset()
+        restore_default()

@@ -16,7 +16,7 @@ def lib():
     except ImportError:
         pytest.skip("libtsqr_amd.so not built")
     yield L
-    autotune.reset()
+    autotune.restore_default()
 
 
 def test_leaf_tiles_table(lib):
@@ -47,9 +47,49 @@ def test_table_argument_checks(lib):
     assert lib.tsqr_set_gemm_waves(1024) == 0
 
 
+def test_tree_arity_table(lib):
+    m = 3 << 20
+    full32 = lib.tsqr_tree_arity(TSQR_FLOAT, m, 32)
+    full16 = lib.tsqr_tree_arity(TSQR_FLOAT, m, 16)
+    full64 = lib.tsqr_tree_arity(TSQR_DOUBLE, m, 32)
+    assert full32 >= 4 and full32 & (full32 - 1) == 0  # H / n with H a power of two
+    assert full16 >= 2 * full32  # a narrower width fits more child Rs into a tile
+    base_ws = lib.tsqr_workspace_bytes(m, 32, TSQR_FLOAT)
+    for a in (2, 4):
+        assert lib.tsqr_set_tree_arity(TSQR_FLOAT, 32, a) == 0
+        assert lib.tsqr_tree_arity(TSQR_FLOAT, m, 32) == a
+        # a narrower tree has more nodes: more blocks in the workspace
+        assert lib.tsqr_workspace_bytes(m, 32, TSQR_FLOAT) > base_ws
+        # other widths and the other dtype keep full blocks
+        assert lib.tsqr_tree_arity(TSQR_FLOAT, m, 16) == full16
+        assert lib.tsqr_tree_arity(TSQR_DOUBLE, m, 32) == full64
+    # above H / n it is clipped to the full block
+    assert lib.tsqr_set_tree_arity(TSQR_FLOAT, 32, 64) == 0
+    assert lib.tsqr_tree_arity(TSQR_FLOAT, m, 32) == full32
+    assert lib.tsqr_set_tree_arity(TSQR_FLOAT, 32, 0) == 0
+    assert lib.tsqr_workspace_bytes(m, 32, TSQR_FLOAT) == base_ws
+
+
+def test_stream_shape_checks(lib):
+    m = 23887872
+    base = lib.tsqr_workspace_bytes(m, 32, TSQR_FLOAT)
+    assert lib.tsqr_set_stream_shape(256, 1024) == 0
+    # fewer streaming workgroups: fewer partial sums to hold
+    assert lib.tsqr_workspace_bytes(m, 32, TSQR_FLOAT) <= base
+    assert lib.tsqr_set_stream_shape(0, 0) == 0
+    assert lib.tsqr_workspace_bytes(m, 32, TSQR_FLOAT) == base
+    assert lib.tsqr_set_stream_shape(1025, 0) == -1
+    assert lib.tsqr_set_stream_shape(-1, 0) == -1
+    assert lib.tsqr_set_stream_shape(0, -5) == -1
+    assert lib.tsqr_set_tree_arity(TSQR_FLOAT, 32, 1) == -1
+    assert lib.tsqr_set_tree_arity(3, 32, 2) == -1
+    assert lib.tsqr_tree_arity(TSQR_FLOAT, 10, 32) == -1
+
+
 def test_save_load_apply_roundtrip(lib, tmp_path):
     table = {"leaf_tiles": {"float32": {"32": {"20": 2, "22": 8}}, "float64": {"16": {"18": 4}}},
-             "gemm_waves": 768}
+             "gemm_waves": 768, "tree_arity": {"float32": {"32": 4}}, "stream_max_groups": 512,
+             "gramq_waves": 4096}
     path = tmp_path / "t.json"
     autotune.save(table, str(path))
     got = autotune.load(str(path))
@@ -58,8 +98,10 @@ def test_save_load_apply_roundtrip(lib, tmp_path):
     assert lib.tsqr_leaf_tiles(TSQR_FLOAT, 3 << 19, 32) == 2
     assert lib.tsqr_leaf_tiles(TSQR_FLOAT, 3 << 21, 32) == 8
     assert lib.tsqr_leaf_tiles(TSQR_DOUBLE, 3 << 17, 16) == 4
+    assert lib.tsqr_tree_arity(TSQR_FLOAT, 1 << 20, 32) == 4
     autotune.reset()
     assert lib.tsqr_leaf_tiles(TSQR_FLOAT, 3 << 21, 32) != 0
+    assert lib.tsqr_tree_arity(TSQR_FLOAT, 1 << 20, 32) > 4
 
 
 def test_committed_table_is_valid():
@@ -74,3 +116,10 @@ def test_committed_table_is_valid():
             assert int(w) in (8, 16, 32)
             for k, tiles in buckets.items():
                 assert 0 <= int(k) <= 31 and tiles in autotune.CANDIDATES
+    for dtype, widths in t.get("tree_arity", {}).items():
+        assert dtype in autotune.DTYPES
+        for w, a in widths.items():
+            assert int(w) in (8, 16, 32) and (a == 0 or a in autotune.ARITY_CANDIDATES)
+    if "stream_max_groups" in t:
+        assert t["stream_max_groups"] in autotune.STREAM_GROUP_CANDIDATES
+        assert t["gramq_waves"] in autotune.GRAMQ_WAVE_CANDIDATES

@@ -115,12 +115,15 @@ def run_random(actors, rng):
 # ---------------------------------------------------------------------------
 
 
-def make_tree(m, n, H, leaf_rows):
+def make_tree(m, n, H, leaf_rows, arity_cap=0):
     """Levels of the reduction tree as plan.hpp builds them: level 0 has
-    tiles of leaf_rows rows, level l > 0 blocks of arity * n rows, each child
-    contributing min(rows, n) rows; stops at the first level with one tile.
-    Returns [(rows, tiles, tile_rows)] per level and the arity."""
+    tiles of leaf_rows rows, level l > 0 blocks of arity * n rows (arity =
+    H / n, or a tuned cap below it), each child contributing min(rows, n)
+    rows; stops at the first level with one tile.  Returns [(rows, tiles,
+    tile_rows)] per level and the arity."""
     arity = H // n
+    if 2 <= arity_cap < arity:
+        arity = arity_cap
     assert arity >= 2
     levels = []
     rows, tile_rows = m, leaf_rows
@@ -198,23 +201,27 @@ def expected_root(levels, arity, start_level):
     return value(len(levels) - 1, 0)
 
 
-# (m, n, H, leaf_rows): ragged last tiles, arity 2 (n = H / 2), wide trees,
-# and the level shapes of real calls (FP32 n = 32: H = 256, arity 8)
+# (m, n, H, leaf_rows, arity cap): ragged last tiles, arity 2 (n = H / 2),
+# wide trees, the level shapes of real calls (FP32 n = 32: H = 256, arity
+# 8), and the tuned narrower trees (cap 2, 3, 4 below H / n)
 TREES = [
-    (27648, 32, 256, 256),
-    (100000, 32, 256, 512),
-    (600000, 32, 256, 1024),
-    (5000, 7, 512, 512),
-    (70001, 20, 256, 256),
-    (9000, 128, 256, 256),
-    (1 << 20, 16, 512, 4096),
+    (27648, 32, 256, 256, 0),
+    (100000, 32, 256, 512, 0),
+    (600000, 32, 256, 1024, 0),
+    (5000, 7, 512, 512, 0),
+    (70001, 20, 256, 256, 0),
+    (9000, 128, 256, 256, 0),
+    (1 << 20, 16, 512, 4096, 0),
+    (100000, 32, 256, 512, 2),
+    (70001, 8, 256, 256, 3),
+    (600000, 32, 256, 1024, 4),
 ]
 
 
 @pytest.mark.parametrize("start_level", [0, 1])
-@pytest.mark.parametrize("m,n,H,leaf_rows", TREES)
-def test_tree_walk_under_random_interleavings(m, n, H, leaf_rows, start_level):
-    levels, arity = make_tree(m, n, H, leaf_rows)
+@pytest.mark.parametrize("m,n,H,leaf_rows,cap", TREES)
+def test_tree_walk_under_random_interleavings(m, n, H, leaf_rows, cap, start_level):
+    levels, arity = make_tree(m, n, H, leaf_rows, cap)
     if len(levels) <= start_level:
         pytest.skip("single-level tree")
     want = expected_root(levels, arity, start_level)
@@ -263,12 +270,12 @@ def stream_group_size(P):  # plan.hpp
     return 8 if P <= 64 else (16 if P <= 256 else 32)
 
 
-def stream_groups(m):  # plan.hpp: workgroups of the streaming Gram for m rows
+def stream_groups(m, cap=256):  # plan.hpp: workgroups of the streaming Gram for m rows
     chunks = (m + 63) // 64
     p = (chunks + 15) // 16
     if p < 256:
         p = min((chunks + 3) // 4, 256)
-    return max(1, min(p, 1024))
+    return max(1, min(p, cap))  # cap: the tuned largest grid, 256 .. 1024
 
 
 def stream_pass(P, partial_values, mem, acquire=True):
@@ -298,9 +305,10 @@ def stream_pass(P, partial_values, mem, acquire=True):
     return [Actor(f"wg{b}", body(b)) for b in range(P)], group_sums
 
 
+@pytest.mark.parametrize("cap", [256, 1000, 1024])
 @pytest.mark.parametrize("m", [27648, 262144, 1 << 20, 23887872])
-def test_stream_group_sums_deterministic(m):
-    P = stream_groups(m)
+def test_stream_group_sums_deterministic(m, cap):
+    P = stream_groups(m, cap)
     G = stream_group_size(P)
     rng0 = random.Random(m)
     # values with wide exponent spread: a different summation order would
@@ -324,7 +332,7 @@ def test_stream_group_sums_deterministic(m):
 
 
 def test_stream_group_without_acquire_is_caught():
-    P = stream_groups(1 << 20)
+    P = stream_groups(1 << 20, 1024)
     vals = [float(i) for i in range(P)]
     caught = 0
     for seed in range(10):

@@ -45,6 +45,9 @@ EXPORTS = (
     "tsqr_set_leaf_tiles",
     "tsqr_leaf_tiles",
     "tsqr_set_gemm_waves",
+    "tsqr_set_tree_arity",
+    "tsqr_tree_arity",
+    "tsqr_set_stream_shape",
     "tsqr_batch_create",
     "tsqr_batch_workspace_bytes",
     "tsqr_batch_run",
@@ -134,6 +137,12 @@ def lib() -> ctypes.CDLL:
     L.tsqr_leaf_tiles.restype = i32
     L.tsqr_set_gemm_waves.argtypes = [i32]
     L.tsqr_set_gemm_waves.restype = i32
+    L.tsqr_set_tree_arity.argtypes = [i32, i32, i32]
+    L.tsqr_set_tree_arity.restype = i32
+    L.tsqr_tree_arity.argtypes = [i32, i32, i32]
+    L.tsqr_tree_arity.restype = i32
+    L.tsqr_set_stream_shape.argtypes = [i32, i32]
+    L.tsqr_set_stream_shape.restype = i32
     for name in ("tsqr_check_s", "tsqr_check_d"):
         f = getattr(L, name)
         f.argtypes = [i32, i32, vp, i32, vp, i32, vp, i32, ctypes.POINTER(ctypes.c_double), vp]

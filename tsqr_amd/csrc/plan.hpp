@@ -46,13 +46,16 @@ struct TreeDesc {
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Builds the tree for an m x n problem: level-0 tiles of leaf_rows rows,
-// blocks of H rows above.  Returns false if the shape cannot be represented
-// (arity < 2 or too many levels).
-inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d) {
+// blocks of H rows above, each holding up to `arity` child Rs: H / n (a
+// full block), or fewer when arity_cap (>= 2) asks for a narrower, deeper
+// tree (the autotuner's knob).  Returns false if the shape cannot be
+// represented (arity < 2 or too many levels).
+inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d, int arity_cap = 0) {
     d->m = m;
     d->n = n;
     d->H = H;
     d->arity = H / n;
+    if (arity_cap >= 2 && arity_cap < d->arity) d->arity = arity_cap;
     if (d->arity < 2) return false;
     int l = 0;
     int64_t rows = m;
@@ -130,6 +133,10 @@ static_assert(stream_group_count(64) <= 32 && stream_group_count(256) <= 32 &&
                   stream_group_count(kStreamMaxGroups) <= 32,
               "at most 32 stream groups");
 
+// Largest grid of the streaming passes, kStreamMaxGroups unless tuned lower
+// (tsqr_set_stream_shape; defined in tsqr_amd.hip).
+int tuned_stream_max_groups();
+
 inline int stream_groups(int64_t m) {
     const int64_t chunks = (m + 63) / 64;
     int64_t p = (chunks + 15) / 16;
@@ -138,7 +145,8 @@ inline int stream_groups(int64_t m) {
         p = one < kStreamFillGroups ? one : kStreamFillGroups;
     }
     if (p < 1) p = 1;
-    return (int)(p < kStreamMaxGroups ? p : kStreamMaxGroups);
+    const int cap = tuned_stream_max_groups();
+    return (int)(p < cap ? p : cap);
 }
 
 inline size_t stream_partials(int64_t m) {

@@ -71,6 +71,23 @@ int tuned_leaf_tiles(int dtype_index, int N, int m) {
 }
 
 namespace {
+std::atomic<int> g_tree_arity[2][3];  // 0: full blocks (H / n)
+// one workgroup of 4 waves per CU measured fastest for the FP64-MFMA-bound
+// R pass on MI355X (23.9M x 32 FP32: 1.853 ms per call at 256, 1.873 at
+// 512, 1.905 at 1024)
+constexpr int kStreamDefaultGroups = 256;
+std::atomic<int> g_stream_max_groups{kStreamDefaultGroups};
+std::atomic<int> g_gramq_waves{2048};
+}  // namespace
+
+int tuned_tree_arity(int dtype_index, int N) {
+    if (dtype_index < 0 || dtype_index > 1) return 0;
+    return g_tree_arity[dtype_index][width_class(N)].load(std::memory_order_relaxed);
+}
+int tuned_stream_max_groups() { return g_stream_max_groups.load(std::memory_order_relaxed); }
+int tuned_gramq_waves() { return g_gramq_waves.load(std::memory_order_relaxed); }
+
+namespace {
 
 struct Pool {
     void* ptr = nullptr;
@@ -888,6 +905,28 @@ int tsqr_leaf_tiles(int dtype, int m, int n) {
     if (dtype == TSQR_FLOAT) return leaf_rows_for<float>(m, n) / (64 * rows_per_lane<float>(N));
     if (dtype == TSQR_DOUBLE) return leaf_rows_for<double>(m, n) / (64 * rows_per_lane<double>(N));
     return TSQR_ERR_ARG;
+}
+
+int tsqr_set_tree_arity(int dtype, int n, int arity) {
+    if ((dtype != TSQR_FLOAT && dtype != TSQR_DOUBLE) || n < 1 || n > 32 || arity < 0 || arity == 1 || arity > 64)
+        return TSQR_ERR_ARG;
+    g_tree_arity[dtype == TSQR_FLOAT ? 0 : 1][width_class(padded_cols(n))].store(arity);
+    return TSQR_OK;
+}
+
+int tsqr_tree_arity(int dtype, int m, int n) {
+    if (n < 1 || n > 32 || m < n) return TSQR_ERR_ARG;
+    TreeDesc d;
+    if (dtype == TSQR_FLOAT) return plan_tsqr<float>(m, n, &d) ? d.arity : TSQR_ERR_SHAPE;
+    if (dtype == TSQR_DOUBLE) return plan_tsqr<double>(m, n, &d) ? d.arity : TSQR_ERR_SHAPE;
+    return TSQR_ERR_ARG;
+}
+
+int tsqr_set_stream_shape(int max_groups, int q_waves) {
+    if (max_groups < 0 || max_groups > kStreamMaxGroups || q_waves < 0 || q_waves > (1 << 16)) return TSQR_ERR_ARG;
+    g_stream_max_groups.store(max_groups ? max_groups : kStreamDefaultGroups);
+    g_gramq_waves.store(q_waves ? q_waves : 2048);
+    return TSQR_OK;
 }
 
 int tsqr_set_gemm_waves(int waves) {

@@ -32,6 +32,11 @@ constexpr int kTargetLeaves = 2048;
 // (dtype index 0 float / 1 double, padded width N, floor(log2 m)); 0 when
 // the bucket has not been tuned.  Defined in tsqr_amd.hip.
 int tuned_leaf_tiles(int dtype_index, int N, int m);
+// Tree arity chosen by the autotuner (tsqr_set_tree_arity) for (dtype index,
+// padded width N); 0 when untuned (full blocks, arity H / n).
+int tuned_tree_arity(int dtype_index, int N);
+// Waves the Gram Q pass aims for when superleaves are few (tsqr_set_stream_shape).
+int tuned_gramq_waves();
 
 // Superleaf height: the tuned number of tiles for this shape if there is
 // one, else about m / kTargetLeaves rows, a whole number of tiles, between
@@ -55,7 +60,7 @@ template <typename T>
 inline bool plan_tsqr(int m, int n, TreeDesc* d) {
     const int N = padded_cols(n);
     const int H = 64 * rows_per_lane<T>(N);
-    return make_tree(m, n, H, leaf_rows_for<T>(m, n), d);
+    return make_tree(m, n, H, leaf_rows_for<T>(m, n), d, tuned_tree_arity(sizeof(T) == 4 ? 0 : 1, N));
 }
 
 // How the level-0 superleaves are factored.

@@ -38,8 +38,6 @@ namespace tsqr {
 // Workgroups of the grid-stride Gram-tree kernels (gram_g, gram_chol):
 // enough to fill the GPU.
 constexpr int kFallbackGrid = 4096;
-// Waves the Q pass of the Gram paths aims for when superleaves are few.
-constexpr int kGramQWaves = 2048;
 // Most superleaves for which a flat-path call walks up the tree inside the
 // superleaf R kernel (tsqr_super_r_up).
 constexpr int kFusedWalkLeaves = 256;
@@ -216,9 +214,10 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     }
     if (gram) {
         // few superleaves: split each into parts so that about
-        // kGramQWaves waves share the rows
+        // tuned_gramq_waves() (default 2048) waves share the rows
         const int tile_blocks = (d.lv[0].tile_rows + 63) / 64;
-        int parts = (kGramQWaves + leaves - 1) / leaves;
+        const int qwaves = tuned_gramq_waves();
+        int parts = (qwaves + leaves - 1) / leaves;
         parts = parts < 1 ? 1 : (parts > tile_blocks ? tile_blocks : parts);
         hipLaunchKernelGGL((tsqr_gram_q<T, VEC>), dim3(leaves * parts), dim3(64), 0, st, d, A, lda,
                            (const double*)side, (const T*)blocks, (const int*)flags, (const int*)ctl, parts);
