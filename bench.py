@@ -439,7 +439,10 @@ def main():
                           "achieved_GBs": round(nbytes / avg_s / 1e9, 1)}
     # Matrix-core work executed per launch.  FP64 (v_mfma_f64_16x16x4f64,
     # 2048 flops each): Gram pass 12 per 16 rows; Q pass 64 per 64 rows, 48
-    # when M = R^-1 is triangular (flat and mixed paths).  BF16
+    # when M = R^-1 is triangular (flat and mixed paths).  The FP32 flat path
+    # runs both passes on v_mfma_f64_4x4x4_4b_f64 (512 flops each): one per
+    # pair of 4-column groups I <= J per 16 rows, i.e. 32 * NC (NC + 1) / 2
+    # flops per row with NC = padded width / 4.  BF16
     # (v_mfma_f32_16x16x32_bf16, 16384 flops each, mixed path): six per 16x16
     # block per 32 rows, three blocks for n > 16, in the Gram pass and again
     # for the residual check fused into the Q pass.
@@ -447,7 +450,13 @@ def main():
     if kind == "tsqr":
         stats = leaf_stats(work.data_ptr(), m, n, code)
         gram_rows = m if stats[1] == 0 else 0
-        f64 = {"leaf_r": gram_rows / 16 * 12 * 2048, "leaf_q": gram_rows / 64 * (48 if stats[2] else 64) * 2048}
+        if code == TSQR_FLOAT and stats[2]:
+            nc = (8 if n <= 8 else (16 if n <= 16 else 32)) // 4
+            per_row = 32 * nc * (nc + 1) // 2
+            f64 = {"leaf_r": gram_rows * per_row, "leaf_q": gram_rows * per_row}
+        else:
+            f64 = {"leaf_r": gram_rows / 16 * 12 * 2048,
+                   "leaf_q": gram_rows / 64 * (48 if stats[2] else 64) * 2048}
         bf16 = {}
     elif kind == "tsqr_mixed":
         mixed = mixed_stats(work.data_ptr(), m, n)

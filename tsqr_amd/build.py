@@ -17,8 +17,10 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtsqr_amd.so")
-SOURCES = [os.path.join(CSRC, f) for f in ("tsqr_f32.hip", "tsqr_f64.hip", "tsqr_amd.hip", "batch_f32.hip",
-                                           "batch_f64.hip", "batch.hip")]
+# the slowest first: every file compiles in a process of its own
+SOURCES = [os.path.join(CSRC, f) for f in ("tsqr_w32s.hip", "tsqr_w32d.hip", "batch_f32.hip", "batch_f64.hip",
+                                           "tsqr_w16s.hip", "tsqr_w16d.hip", "tsqr_w8s.hip", "tsqr_w8d.hip",
+                                           "tsqr_f32.hip", "tsqr_f64.hip", "tsqr_amd.hip", "batch.hip")]
 OBJDIR = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 

@@ -1,9 +1,11 @@
-// tsqr_launch_impl.hpp -- body of tsqr_launch<T>, included once per dtype.
+// tsqr_launch_impl.hpp -- body of tsqr_launch<T>: the dispatcher is
+// instantiated in tsqr_f32.hip / tsqr_f64.hip, the launch sequence per
+// (dtype, padded width) in tsqr_w{8,16,32}{s,d}.hip.
 //
 // Launch sequence for one call (m > H):
 //   init          tickets = 0, superleaf flags = path, control words = 0
 //   gram_stream   (auto path) FP64 Gram matrix of the whole matrix by a
-//                 persistent grid, its Cholesky and the gate in the same
+//                 persistent grid (FP32: gram_stream4, on 4 x 4 tiles), its Cholesky and the gate in the same
 //                 launch; if the whole matrix passes the gate this is the
 //                 flat Gram path (Q = A R^{-1}) and the per-superleaf kernels
 //                 and the tree below return at once
@@ -15,6 +17,7 @@
 //   tree_up       levels >= 1, one launch, ticketed walk to the root
 //   down          levels root-1 .. 1, one launch each
 //   gram_q        M = R^{-1} S, then Q = A M, for the Gram superleaves
+//                 (FP32 flat path: flat_q4, Q = A R^{-1} by a persistent grid)
 //   super_q       Householder Q for the flagged superleaves
 // The CholeskyQR2 path replaces the Gram kernels by mixed_gram (FP32,
 // BF16x3) with the flat reduction and Cholesky, or gram_stream (FP64), then mixed_q +
@@ -30,6 +33,7 @@
 #include "../../include/tsqr_amd.h"
 #include "check.hpp"
 #include "mixed.hpp"
+#include "stream4.hpp"
 #include "super.hpp"
 #include "tsqr_launch.hpp"
 
@@ -154,8 +158,12 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
         }
     }
     if (flat_auto) {
-        hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
-                           partial, stream_tk);
+        if constexpr (sizeof(T) == 4)
+            hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, true>), dim3(sgroups), dim3(256), 0, st, d.m, d.n,
+                               (const float*)A, lda, partial, stream_tk);
+        else
+            hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
+                               partial, stream_tk);
         hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
                            (const double*)stream_sum, R, ldr, side, ctl, o.gate, (double*)nullptr);
     }
@@ -212,7 +220,13 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
                                o.tol);
         }
     }
-    if (gram) {
+    if (flat_auto && sizeof(T) == 4) {
+        // FP32 flat path: Q = A R^{-1} by a persistent grid of the streaming
+        // passes' size (nothing if the gate rejected the matrix)
+        if constexpr (sizeof(T) == 4)
+            hipLaunchKernelGGL((tsqr_flat_q4<N / 4, VEC, true>), dim3(sgroups), dim3(256), 0, st, d.m, d.n,
+                               (float*)A, lda, X0, (const int*)ctl);
+    } else if (gram) {
         // few superleaves: split each into parts so that about
         // tuned_gramq_waves() (default 2048) waves share the rows
         const int tile_blocks = (d.lv[0].tile_rows + 63) / 64;
@@ -238,6 +252,17 @@ int launch_tsqr_v(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     if (aligned16(A, lda, sizeof(T))) return launch_tsqr_n<T, N, true>(d, A, lda, R, ldr, work, o, st);
     return launch_tsqr_n<T, N, false>(d, A, lda, R, ldr, work, o, st);
 }
+
+// Each (dtype, padded width) pair is instantiated in a translation unit of
+// its own (tsqr_w*.hip), so that the six compile in parallel.
+#define TSQR_WIDTH_UNIT(T, N)                                                                                   \
+    template int launch_tsqr_v<T, N>(const TreeDesc&, T*, int, T*, int, void*, const LaunchOpts&, hipStream_t)
+extern TSQR_WIDTH_UNIT(float, 8);
+extern TSQR_WIDTH_UNIT(float, 16);
+extern TSQR_WIDTH_UNIT(float, 32);
+extern TSQR_WIDTH_UNIT(double, 8);
+extern TSQR_WIDTH_UNIT(double, 16);
+extern TSQR_WIDTH_UNIT(double, 32);
 
 template <typename T>
 int tsqr_launch(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, const LaunchOpts& o, hipStream_t st) {

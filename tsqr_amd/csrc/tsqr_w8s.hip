@@ -1,0 +1,6 @@
+// TSQR launch sequence (tsqr_launch_impl.hpp) for float matrices of padded width 8.
+#include "tsqr_launch_impl.hpp"
+
+namespace tsqr {
+TSQR_WIDTH_UNIT(float, 8);
+}
