@@ -411,7 +411,7 @@ __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restric
     if (first >= leaf_rows) return;
     // ---- M = X S ----
     // flat path: X = R^{-1} of the whole matrix (side block T0) and S = I
-    const double* X = side + (int64_t)(flat ? L0.tiles : leaf) * 1024;  // row-major
+    const double* X = side + (int64_t)(flat ? d.side_leaves : leaf) * 1024;  // row-major
     const bool root = flat || L0.tiles == 1;
     const T* S = nullptr;
     if (!root) {
@@ -586,16 +586,6 @@ __device__ __forceinline__ void sum_partials(GramLds& L, int nparts, const doubl
     }
 }
 
-// 1/sqrt(x) for x > 0 finite: the hardware estimate and two Newton steps
-// (y <- y + y (1/2 - (x/2) y^2)), within a few ulp of the rounded value.
-__device__ __forceinline__ double rsqrt_fp64(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double h = 0.5 * x;
-#pragma unroll
-    for (int i = 0; i < 2; ++i) y = __builtin_fma(y, __builtin_fma(-h * y, y, 0.5), y);
-    return y;
-}
-
 // Cholesky factor and its inverse in one sweep, by one wave.  Lane c < 32
 // starts from column c of the Gram matrix in L.G (identity outside the
 // leading n x n block) and ends with column c of R (zeros below the
@@ -685,7 +675,7 @@ __global__ __launch_bounds__(kSumThreads) void tsqr_gram_flat(const TreeDesc d, 
     sum_partials(L, nparts, partial, t);
     __syncthreads();
     if (t >= 64) return;
-    flat_factor<T>(L, d.n, R, ldr, side + (int64_t)d.lv[0].tiles * 1024, ctl, gate, rkeep, t);
+    flat_factor<T>(L, d.n, R, ldr, side + (int64_t)d.side_leaves * 1024, ctl, gate, rkeep, t);
 }
 
 // ---------------------------------------------------------------------------

@@ -415,7 +415,7 @@ __global__ __launch_bounds__(64) void tsqr_mixed_q(const TreeDesc d, T* __restri
                                                    double* __restrict__ side, const int* __restrict__ ctl, int pass) {
     __shared__ double sums[12 * 64];
     if (!ctl[0] || (pass == 2 && !ctl[1])) return;
-    mixed_q_body<T, NT, VEC>(d.lv[0], d.n, A, lda, side + (int64_t)d.lv[0].tiles * 1024,
+    mixed_q_body<T, NT, VEC>(d.lv[0], d.n, A, lda, side + (int64_t)d.side_leaves * 1024,
                              side + (int64_t)blockIdx.x * 1024, sums, blockIdx.x);
 }
 
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(kSumThreads) void tsqr_mixed_check(const TreeDesc d
     sum_partials(L, nparts, partial, t);
     __syncthreads();
     if (t >= 64) return;
-    const int64_t T0 = d.lv[0].tiles;
+    const int64_t T0 = d.side_leaves;
     mixed_check_body<T>(L, d.n, R, ldr, side + T0 * 1024, side + (T0 + 1) * 1024, side + (T0 + 2) * 1024, ctl,
                         pass, tol, t);
 }

@@ -39,6 +39,10 @@ struct TreeDesc {
     int arity;   // children per parent
     int levels;  // number of levels (root is levels - 1)
     int total_ctr;
+    // FP64 32 x 32 side blocks kept per level-0 superleaf (the superleaf
+    // Gram paths); 0 for plans whose leaves are single Householder tiles
+    // (tile_leaf_plan), which need only the whole-matrix blocks after them
+    int side_leaves;
     int64_t block_elems;  // elements of all blocks
     LevelDesc lv[kMaxLevels];
 };
@@ -86,6 +90,7 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d, int arity
     }
     d->levels = l + 1;
     d->total_ctr = ctr;
+    d->side_leaves = d->lv[0].tiles;
     d->block_elems = blk;
     return true;
 }
@@ -104,9 +109,10 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d, int arity
 //            stage when there are more than kGramPartials2 of them, or the
 //            streaming Gram kernel's P + ceil(P / 32) blocks (P =
 //            stream_partials(m)), whichever is more
-//   side     32 x 32 FP64 per superleaf (the Gram matrix, then R^{-1}), then
-//            three more: the whole matrix's R^{-1} on the flat path (T0),
-//            its R (T0+1) and the mixed path's residuals (T0+2)
+//   side     32 x 32 FP64 per superleaf (the Gram matrix, then R^{-1}; none
+//            for single-tile leaves: side_leaves), then three more: the
+//            whole matrix's R^{-1} on the flat path (T0 = side_leaves), its
+//            R (T0+1) and the mixed path's residuals (T0+2)
 constexpr int kGramPartials = 1024;
 constexpr int kGramPartials2 = 32;
 constexpr int kCtlWords = 192;          // control words, including the stream tickets
@@ -149,9 +155,45 @@ inline int stream_groups(int64_t m) {
     return (int)(p < cap ? p : cap);
 }
 
+// The FP32 flat path's passes on 4 x 4 tiles (stream4.hpp).  The R pass
+// (tsqr_gram_stream4) takes over from tsqr_gram_stream at kStream4MinRows
+// rows: below that its longer reduction at the end cost more than its
+// cheaper products saved (27648 x 32: 23 against 14 us; 1M x 32: 56
+// against 70 us).  Its grid doubles the cap (two workgroups per CU) once
+// every wave has at least 64 chunks of 64 rows (23.9M x 32: 0.515 against
+// 0.552 ms; 4.2M x 32: 0.131 against 0.121 ms, so not there).  The last
+// kCachedChunks chunks of its sweep are read with plain loads, so that they
+// stay in the memory-side cache (256 MB on MI355X) for the Q pass.
+constexpr int64_t kStream4MinRows = (int64_t)1 << 19;
+constexpr int kCachedChunks = 24576;  // 192 MB of 32 FP32 columns
+inline bool use_stream4(int64_t m) { return m >= kStream4MinRows; }
+inline int stream_groups4(int64_t m) {
+    const int64_t chunks = m / 64;
+    const int cap = tuned_stream_max_groups();
+    const int big = 2 * cap < kStreamMaxGroups ? 2 * cap : kStreamMaxGroups;
+    return chunks >= (int64_t)64 * 4 * big ? big : stream_groups(m);
+}
+// Grid of the FP32 flat Q pass (tsqr_flat_q4): one chunk per wave up to the
+// cap, or up to twice the cap below 2048 chunks (measured: 100000 x 32 runs
+// fastest with 391 workgroups, 262144 x 32 and taller with 256).
+inline int flat_q_groups(int64_t m) {
+    const int64_t chunks = (m + 63) / 64;
+    const int cap = tuned_stream_max_groups();
+    int64_t lim = chunks < 2048 ? 2 * (int64_t)cap : cap;
+    if (lim > kStreamMaxGroups) lim = kStreamMaxGroups;
+    int64_t p = (chunks + 3) / 4;
+    if (p > lim) p = lim;
+    return (int)(p < 1 ? 1 : p);
+}
+
+// Partial blocks of a streaming Gram launch of P workgroups: one per
+// workgroup plus one per group; the workspace holds enough for either grid.
 inline size_t stream_partials(int64_t m) {
     const int p = stream_groups(m);
-    return (size_t)(p + stream_group_count(p));
+    const int p4 = stream_groups4(m);
+    const size_t a = (size_t)(p + stream_group_count(p));
+    const size_t b = (size_t)(p4 + stream_group_count(p4));
+    return a > b ? a : b;
 }
 
 // Workspace of the device-side checker (check.hpp): three partial areas of
@@ -173,12 +215,13 @@ inline WsLayout ws_layout(const TreeDesc& d, size_t elem) {
     w.list = w.flags + align256(leaves * sizeof(int));
     w.ctl = w.list + align256((leaves + 1) * sizeof(int));
     w.partial = w.ctl + kCtlWords * sizeof(int);
-    const size_t parts = leaves < (size_t)kGramPartials ? leaves : (size_t)kGramPartials;
+    const size_t sl = (size_t)d.side_leaves;
+    const size_t parts = sl < (size_t)kGramPartials ? sl : (size_t)kGramPartials;
     const size_t parts2 = parts > (size_t)kGramPartials2 ? (size_t)kGramPartials2 : 0;
     const size_t sparts = stream_partials(d.m);
     const size_t nparts = parts + parts2 > sparts ? parts + parts2 : sparts;
     w.side = w.partial + nparts * 1024 * sizeof(double);
-    w.total = w.side + align256((leaves + 3) * 1024 * sizeof(double));
+    w.total = w.side + align256((sl + 3) * 1024 * sizeof(double));
     return w;
 }
 

@@ -79,12 +79,17 @@ __device__ __forceinline__ void factor_step(T (&x)[RPL][N], TileLds<T, N>& L, in
     }
     wave_sync();
 
+    // nu = sqrt(djj) and gamma = 1/sqrt(den) from the reciprocal square
+    // root (two Newton steps): a shorter dependent chain than a rounded
+    // square root and a division, and this chain is on every reflector's
+    // critical path.  A zero column gives nu = 0 and gamma = 0; a NaN
+    // propagates.
     const double djj = (double)L.d[0];
     const double xjj = (double)L.row[J];
-    const double nu = __builtin_sqrt(djj);
+    const double nu = djj * (djj > 0.0 ? rsqrt_fp64(djj) : 0.0);
     const double snu = (xjj >= 0.0) ? nu : -nu;
     const double den = nu * (nu + __builtin_fabs(xjj));
-    const double gamma = den > 0.0 ? 1.0 / __builtin_sqrt(den) : 0.0;
+    const double gamma = den > 0.0 ? rsqrt_fp64(den) : 0.0;
     if (mine) {
         const double xjc = (double)L.row[J + cl];
         if (cl == 0) {
@@ -114,11 +119,25 @@ __device__ __forceinline__ void factor_step(T (&x)[RPL][N], TileLds<T, N>& L, in
     wave_sync();
 }
 
-template <typename T, int N, int RPL, int J = 0>
-__device__ __forceinline__ void factor_tile(T (&x)[RPL][N], TileLds<T, N>& L, int lane) {
+// No-op column sink of factor_tile.
+struct NoColumnSink {
+    template <int J, typename T, int RPL>
+    __device__ __forceinline__ void column(const T (&)[RPL]) const {}
+};
+
+// sink.column<J>(u) receives this lane's rows of reflector J as soon as it
+// is final, so a caller that keeps the reflectors (in memory) stores each
+// column while the rest of the tile is factored and its registers are free
+// from then on.
+template <typename T, int N, int RPL, int J = 0, typename Sink = NoColumnSink>
+__device__ __forceinline__ void factor_tile(T (&x)[RPL][N], TileLds<T, N>& L, int lane, const Sink& sink = Sink{}) {
     if constexpr (J < N) {
         factor_step<T, N, RPL, J>(x, L, lane);
-        factor_tile<T, N, RPL, J + 1>(x, L, lane);
+        T u[RPL];
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) u[s] = x[s][J];
+        sink.template column<J, T, RPL>(u);
+        factor_tile<T, N, RPL, J + 1, Sink>(x, L, lane, sink);
     }
 }
 

@@ -139,9 +139,12 @@ int check_args(int m, int n, const T* A, int lda, const T* R, int ldr) {
 template <typename T>
 size_t tsqr_ws_bytes(int m, int n) {
     if (n < 1 || m < n || n > 32) return 0;
-    TreeDesc d;
-    if (!plan_tsqr<T>(m, n, &d)) return 0;
-    return tree_workspace_bytes(d, sizeof(T));
+    // enough for either plan (single-tile leaves or superleaves), so the
+    // size does not depend on the path a call takes
+    TreeDesc d, e;
+    if (!plan_tsqr<T>(m, n, &d, false) || !plan_tsqr<T>(m, n, &e, true)) return 0;
+    const size_t a = tree_workspace_bytes(d, sizeof(T)), b = tree_workspace_bytes(e, sizeof(T));
+    return a > b ? a : b;
 }
 
 // Leaf path per dtype (tsqr_set_leaf_path).  FP32 defaults to the Gram path
@@ -173,9 +176,9 @@ LaunchOpts mixed_opts() {
 template <typename T>
 int tsqr_core(int m, int n, T* A, int lda, T* R, int ldr, void* work, hipStream_t st, bool mixed = false,
               bool r_only = false, bool untimed = false) {
-    TreeDesc d;
-    if (!plan_tsqr<T>(m, n, &d)) return TSQR_ERR_SHAPE;
     LaunchOpts o = mixed ? mixed_opts() : current_opts<T>();
+    TreeDesc d;
+    if (!plan_tsqr<T>(m, n, &d, !mixed && tile_leaf_path(o.path))) return TSQR_ERR_SHAPE;
     o.r_only = r_only ? 1 : 0;
     o.untimed = untimed ? 1 : 0;
     return tsqr_launch<T>(d, A, lda, R, ldr, work, o, st);
@@ -830,7 +833,8 @@ int tsqr_leaf_stats(const void* work, int m, int n, int dtype, int* gram_leaves,
     if (n < 1 || m < n || n > 32) return TSQR_ERR_SHAPE;
     TreeDesc d;
     const size_t elem = dtype == TSQR_FLOAT ? sizeof(float) : sizeof(double);
-    const bool ok = dtype == TSQR_FLOAT ? plan_tsqr<float>(m, n, &d) : plan_tsqr<double>(m, n, &d);
+    const bool tiles = tile_leaf_path(dtype == TSQR_FLOAT ? current_opts<float>().path : current_opts<double>().path);
+    const bool ok = dtype == TSQR_FLOAT ? plan_tsqr<float>(m, n, &d, tiles) : plan_tsqr<double>(m, n, &d, tiles);
     if (!ok) return TSQR_ERR_SHAPE;
     *gram_leaves = 0;
     *householder_leaves = 0;
@@ -879,7 +883,7 @@ int tsqr_mixed_stats(const void* work, int m, int n, int* flat, int* passes, int
     const char* base = reinterpret_cast<const char*>(work);
     if (hipDeviceSynchronize() != hipSuccess ||
         hipMemcpy(ctl, base + w.ctl, sizeof(ctl), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(e, base + w.side + (size_t)(d.lv[0].tiles + 2) * 1024 * sizeof(double), sizeof(e),
+        hipMemcpy(e, base + w.side + (size_t)(d.side_leaves + 2) * 1024 * sizeof(double), sizeof(e),
                   hipMemcpyDeviceToHost) != hipSuccess)
         return TSQR_ERR_HIP;
     *flat = ctl[0];
@@ -917,8 +921,10 @@ int tsqr_set_tree_arity(int dtype, int n, int arity) {
 int tsqr_tree_arity(int dtype, int m, int n) {
     if (n < 1 || n > 32 || m < n) return TSQR_ERR_ARG;
     TreeDesc d;
-    if (dtype == TSQR_FLOAT) return plan_tsqr<float>(m, n, &d) ? d.arity : TSQR_ERR_SHAPE;
-    if (dtype == TSQR_DOUBLE) return plan_tsqr<double>(m, n, &d) ? d.arity : TSQR_ERR_SHAPE;
+    if (dtype == TSQR_FLOAT)
+        return plan_tsqr<float>(m, n, &d, tile_leaf_path(current_opts<float>().path)) ? d.arity : TSQR_ERR_SHAPE;
+    if (dtype == TSQR_DOUBLE)
+        return plan_tsqr<double>(m, n, &d, tile_leaf_path(current_opts<double>().path)) ? d.arity : TSQR_ERR_SHAPE;
     return TSQR_ERR_ARG;
 }
 

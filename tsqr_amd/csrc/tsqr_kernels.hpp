@@ -213,13 +213,52 @@ __device__ __forceinline__ void tree_up_body(const TreeDesc& d, T* __restrict__ 
     store_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
 }
 
+// A grid-stride loop over the level-1 blocks, so that a small grid serves
+// any tree (and costs one short launch when skip[0] is set).
 template <typename T, int N, int RPL, bool FORMQ = false>
 __global__ __launch_bounds__(64) void tsqr_tree_up(const TreeDesc d, T* __restrict__ R, int ldr,
                                                     T* __restrict__ blocks, int* __restrict__ tickets,
                                                     const int* __restrict__ skip) {
     __shared__ TileLds<T, N> L;
     if (skip[0]) return;
-    tree_up_body<T, N, RPL, FORMQ>(d, R, ldr, blocks, tickets, L, blockIdx.x);
+#pragma unroll 1
+    for (int tile = blockIdx.x; tile < d.lv[1].tiles; tile += gridDim.x)
+        tree_up_body<T, N, RPL, FORMQ>(d, R, ldr, blocks, tickets, L, tile);
+}
+
+// One tree level per launch, no tickets (the pure Householder path of tall
+// matrices): block `tile` of `level` is factored; its R goes to the parent
+// block, its reflectors (or, with FORMQ, its explicit Q) stay over its block.
+// The root writes R to the caller and its explicit Q over its block, as
+// tree_up_body does.  Kernel boundaries order the levels, so no
+// agent-scope release per block is needed (tree_up_body needs one per
+// block: an L2 write-back on MI355X).
+template <typename T, int N, int RPL, bool FORMQ = false>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void tsqr_tree_level(const TreeDesc d, int level, T* __restrict__ R, int ldr,
+                                                       T* __restrict__ blocks, const int* __restrict__ skip) {
+    __shared__ TileLds<T, N> L;
+    if (skip[0]) return;
+    const int lane = threadIdx.x;
+    const int n = d.n;
+    const LevelDesc& Lv = d.lv[level];
+    const int tile = blockIdx.x;
+    const int rows = tile_rows_of(Lv, tile);
+    T* blk = blocks + Lv.blk_off + (int64_t)tile * d.H * n;
+    T x[RPL][N];
+    load_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
+    factor_tile<T, N, RPL>(x, L, lane);
+    if (Lv.tiles == 1) {
+        store_r<T, N>(L, R, ldr, n, n, lane);
+        form_q_tile<T, N, RPL>(x, L, lane);
+    } else {
+        const LevelDesc& P = d.lv[level + 1];
+        const int parent = tile / d.arity;
+        const int slot = tile - parent * d.arity;
+        store_r<T, N>(L, blocks + P.blk_off + (int64_t)parent * d.H * n + slot * n, d.H, rows < n ? rows : n, n,
+                      lane);
+        if constexpr (FORMQ) form_q_tile<T, N, RPL>(x, L, lane);
+    }
+    store_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
 }
 
 // S of superleaf `leaf` after a FORMQ up sweep: B_1 B_2 ... B_top, B_l the

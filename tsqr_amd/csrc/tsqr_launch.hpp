@@ -56,11 +56,17 @@ inline int leaf_rows_for(int m, int n) {
     return (int)tiles * H;
 }
 
+// tile_leaves: level-0 leaves of one tile each and no per-leaf side blocks
+// (the paths whose leaves are Householder tiles only, tile_leaf_path);
+// otherwise superleaves of leaf_rows_for rows for the superleaf Gram paths.
 template <typename T>
-inline bool plan_tsqr(int m, int n, TreeDesc* d) {
+inline bool plan_tsqr(int m, int n, TreeDesc* d, bool tile_leaves = false) {
     const int N = padded_cols(n);
     const int H = 64 * rows_per_lane<T>(N);
-    return make_tree(m, n, H, leaf_rows_for<T>(m, n), d, tuned_tree_arity(sizeof(T) == 4 ? 0 : 1, N));
+    if (!make_tree(m, n, H, tile_leaves ? H : leaf_rows_for<T>(m, n), d, tuned_tree_arity(sizeof(T) == 4 ? 0 : 1, N)))
+        return false;
+    if (tile_leaves) d->side_leaves = 0;
+    return true;
 }
 
 // How the level-0 superleaves are factored.
@@ -72,6 +78,11 @@ enum LeafPath {
                            // (mixed.hpp): BF16x3 Gram for FP32 (the mixed-precision method),
                            // FP64 MFMA Gram for FP64; Householder tree if the gate rejects
 };
+
+// Paths planned with single-tile leaves: their level-0 work per tile is the
+// Householder kernels of leaf1.hpp (AUTO's fallback, HOUSEHOLDER); the flat
+// Gram passes of AUTO do not depend on the leaves.
+inline bool tile_leaf_path(int path) { return path == kLeafAuto || path == kLeafHouseholder; }
 
 struct LaunchOpts {
     int path;        // LeafPath

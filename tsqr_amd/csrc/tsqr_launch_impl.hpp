@@ -32,6 +32,7 @@
 #pragma once
 #include "../../include/tsqr_amd.h"
 #include "check.hpp"
+#include "leaf1.hpp"
 #include "mixed.hpp"
 #include "stream4.hpp"
 #include "super.hpp"
@@ -45,6 +46,10 @@ constexpr int kFallbackGrid = 4096;
 // Most superleaves for which a flat-path call walks up the tree inside the
 // superleaf R kernel (tsqr_super_r_up).
 constexpr int kFusedWalkLeaves = 256;
+// Grid of the single-tile leaf kernels (leaf1.hpp) where they only back up
+// a flat path: small, so that the launch costs little when they return at
+// once; the pure Householder path launches one workgroup per tile.
+constexpr int kFallbackTileGrid = 2048;
 
 namespace {
 // tickets = 0; flags = value; the Householder list is empty (value 0) or
@@ -126,14 +131,14 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
                        d.total_ctr, flags, list, leaves, force, ctl);
     // Householder superleaf kernels: a grid-stride loop over the list
     const int super_grid = leaves < 1024 ? leaves : 1024;
-    double* rkeep = side + (size_t)(leaves + 1) * 1024;
+    double* rkeep = side + (size_t)(d.side_leaves + 1) * 1024;
     if (timed) timing_mark(kPhaseLeafR, true, st);
     // streaming Gram: sgroups workgroups, then sgroup_parts group partials
     int* stream_tk = ctl + kCtlStreamTickets;
     const int sgroups = stream_groups(d.m);
     const int sgroup_parts = stream_group_count(sgroups);
     double* stream_sum = partial + (size_t)sgroups * 1024;
-    const double* X0 = side + (size_t)leaves * 1024;  // R^{-1} of the flat path
+    const double* X0 = side + (size_t)d.side_leaves * 1024;  // R^{-1} of the flat path
     if (mixed) {
         if constexpr (sizeof(T) == 4) {
             hipLaunchKernelGGL((tsqr_mixed_stream<NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const float*)A, lda,
@@ -158,14 +163,26 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
         }
     }
     if (flat_auto) {
-        if constexpr (sizeof(T) == 4)
-            hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, true>), dim3(sgroups), dim3(256), 0, st, d.m, d.n,
-                               (const float*)A, lda, partial, stream_tk);
-        else
+        // FP32 from kStream4MinRows rows: the R pass on 4 x 4 tiles, with its
+        // own grid
+        const bool four = sizeof(T) == 4 && use_stream4(d.m);
+        const int rgroups = four ? stream_groups4(d.m) : sgroups;
+        if (four) {
+            if constexpr (sizeof(T) == 4) {
+                if (d.n == N)
+                    hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, true>), dim3(rgroups), dim3(256), 0, st, d.m,
+                                       d.n, (const float*)A, lda, partial, stream_tk, kCachedChunks);
+                else
+                    hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, false>), dim3(rgroups), dim3(256), 0, st, d.m,
+                                       d.n, (const float*)A, lda, partial, stream_tk, kCachedChunks);
+            }
+        } else {
             hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
                                partial, stream_tk);
-        hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
-                           (const double*)stream_sum, R, ldr, side, ctl, o.gate, (double*)nullptr);
+        }
+        hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, stream_group_count(rgroups),
+                           (const double*)(partial + (size_t)rgroups * 1024), R, ldr, side, ctl, o.gate,
+                           (double*)nullptr);
     }
     if (gram_tree) {
         const int gram_grid = leaves < kFallbackGrid ? leaves : kFallbackGrid;
@@ -181,11 +198,31 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     // nearly never runs, and the superleaf R kernel walks up the tree itself
     // (tsqr_super_r_up).
     const bool flat_first = flat_auto || mixed;
+    // single-tile leaves: one wave per tile, the R kernel always walks up
+    const bool tile_leaves = d.side_leaves == 0 && d.lv[0].tile_rows == d.H;
+    const int tile_grid = flat_first ? (leaves < kFallbackTileGrid ? leaves : kFallbackTileGrid) : leaves;
     // The fused walk saves a launch, which matters on short calls; on tall
     // matrices it measured slower (one wave walks while the rest of its
     // workgroup idles), and they keep the separate tree launch.
-    const bool fused_up = flat_first && d.levels >= 2 && leaves <= kFusedWalkLeaves;
-    if (fused_up)
+    const bool fused_up = (tile_leaves || flat_first) && d.levels >= 2 && leaves <= kFusedWalkLeaves;
+    if (tile_leaves) {
+        // the reflectors go over A unless A must stay as it is (R only)
+        auto leaf_r = [&](auto walk) {
+            constexpr int WK = decltype(walk)::value;
+            if (o.r_only)
+                hipLaunchKernelGGL((tsqr_leaf_r1<T, N, RPL, VEC, WK, false>), dim3(tile_grid), dim3(64), 0, st, d, A,
+                                   lda, R, ldr, blocks, tickets, (const int*)list, (const int*)ctl);
+            else
+                hipLaunchKernelGGL((tsqr_leaf_r1<T, N, RPL, VEC, WK, true>), dim3(tile_grid), dim3(64), 0, st, d, A,
+                                   lda, R, ldr, blocks, tickets, (const int*)list, (const int*)ctl);
+        };
+        if (fused_up && flat_first)
+            leaf_r(std::integral_constant<int, 2>{});
+        else if (fused_up)
+            leaf_r(std::integral_constant<int, 1>{});
+        else
+            leaf_r(std::integral_constant<int, 0>{});
+    } else if (fused_up)
         hipLaunchKernelGGL((tsqr_super_r_up<T, N, RPL, VEC>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0, st,
                            d, (const T*)A, lda, R, ldr, blocks, tickets, (const int*)list, (const int*)ctl);
     else
@@ -194,11 +231,20 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     if (timed) timing_mark(kPhaseLeafR, false, st);
     if (d.levels >= 2 && !fused_up) {
         if (timed) timing_mark(kPhaseTree, true, st);
+        // where the tree only backs up a flat path, a small grid-stride grid
+        const int l1 = d.lv[1].tiles;
+        const int tree_grid = flat_first && l1 > kFallbackTileGrid ? kFallbackTileGrid : l1;
         if (flat_first)
-            hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, true>), dim3(d.lv[1].tiles), dim3(64), 0, st, d, R, ldr,
+            hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, true>), dim3(tree_grid), dim3(64), 0, st, d, R, ldr,
                                blocks, tickets, (const int*)ctl);
+        else if (tile_leaves)
+            // pure Householder over single-tile leaves: one launch per level,
+            // no per-block tickets (the down sweep is queued anyway)
+            for (int l = 1; l < d.levels; ++l)
+                hipLaunchKernelGGL((tsqr_tree_level<T, N, RPL, false>), dim3(d.lv[l].tiles), dim3(64), 0, st, d,
+                                   l, R, ldr, blocks, (const int*)ctl);
         else
-            hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, false>), dim3(d.lv[1].tiles), dim3(64), 0, st, d, R, ldr,
+            hipLaunchKernelGGL((tsqr_tree_up<T, N, RPL, false>), dim3(tree_grid), dim3(64), 0, st, d, R, ldr,
                                blocks, tickets, (const int*)ctl);
         if (timed) timing_mark(kPhaseTree, false, st);
     }
@@ -221,11 +267,11 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
         }
     }
     if (flat_auto && sizeof(T) == 4) {
-        // FP32 flat path: Q = A R^{-1} by a persistent grid of the streaming
-        // passes' size (nothing if the gate rejected the matrix)
+        // FP32 flat path: Q = A R^{-1} by a persistent grid (nothing if the
+        // gate rejected the matrix)
         if constexpr (sizeof(T) == 4)
-            hipLaunchKernelGGL((tsqr_flat_q4<N / 4, VEC, true>), dim3(sgroups), dim3(256), 0, st, d.m, d.n,
-                               (float*)A, lda, X0, (const int*)ctl);
+            hipLaunchKernelGGL((tsqr_flat_q4<N / 4, VEC, true>), dim3(flat_q_groups(d.m)), dim3(256), 0, st, d.m,
+                               d.n, (float*)A, lda, X0, (const int*)ctl);
     } else if (gram) {
         // few superleaves: split each into parts so that about
         // tuned_gramq_waves() (default 2048) waves share the rows
@@ -236,7 +282,13 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
         hipLaunchKernelGGL((tsqr_gram_q<T, VEC>), dim3(leaves * parts), dim3(64), 0, st, d, A, lda,
                            (const double*)side, (const T*)blocks, (const int*)flags, (const int*)ctl, parts);
     }
-    if (flat_first)
+    if (tile_leaves && flat_first)
+        hipLaunchKernelGGL((tsqr_leaf_q1<T, N, RPL, VEC, true>), dim3(tile_grid), dim3(64), 0, st, d, A, lda,
+                           (const T*)blocks, (const int*)list, (const int*)ctl);
+    else if (tile_leaves)
+        hipLaunchKernelGGL((tsqr_leaf_q1<T, N, RPL, VEC, false>), dim3(tile_grid), dim3(64), 0, st, d, A, lda,
+                           (const T*)blocks, (const int*)list, (const int*)ctl);
+    else if (flat_first)
         hipLaunchKernelGGL((tsqr_super_q<T, N, RPL, VEC, true>), dim3(super_grid), dim3(64 * super_tiles<T>(N)), 0,
                            st, d, A, lda, (const T*)blocks, (const int*)list, (const int*)ctl);
     else

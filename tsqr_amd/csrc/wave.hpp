@@ -187,6 +187,16 @@ __device__ __forceinline__ double readlane_t<double>(double v, int lane) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// 1/sqrt(x) for x > 0 finite: the hardware estimate and two Newton steps
+// (y <- y + y (1/2 - (x/2) y^2)), within a few ulp of the rounded value.
+__device__ __forceinline__ double rsqrt_fp64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) y = __builtin_fma(y, __builtin_fma(-h * y, y, 0.5), y);
+    return y;
+}
+
 // Orders LDS traffic between the lanes of one wavefront.  LDS instructions of
 // a wave execute in order, so no hardware barrier is needed; the fences keep
 // the compiler from moving memory operations across this point, and it must
