@@ -33,7 +33,12 @@ template <typename T>
 bool make_prob(int m, int n, void* A, int lda, void* R, int ldr, int method, size_t* offset, BatchProb* p) {
     const int N = padded_cols(n);
     const int H = 64 * rows_per_lane<T>(N);
-    if (!make_tree(m, n, H, super_tiles<T>(N) * H, &p->d)) return false;
+    // plain (Householder) problems: leaves of one tile, one wave each
+    // (batch_leaf_r / batch_leaf_q), and no per-leaf side blocks; mixed
+    // problems: superleaves of super_tiles tiles for the per-superleaf Grams
+    const bool plain = method != TSQR_BATCH_MIXED;
+    if (!make_tree(m, n, H, (plain ? 1 : super_tiles<T>(N)) * H, &p->d)) return false;
+    if (plain) p->d.side_leaves = 0;
     p->w = ws_layout(p->d, sizeof(T));
     p->ws = (int64_t)*offset;
     *offset += align256(p->w.total);
@@ -79,7 +84,9 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
             c.n2 = n2;
             c.lda = lda[i];
             c.ldr = ldr[i];
-            c.leaf_rows = p.d.lv[0].tile_rows;
+            // row blocks of the coupling products: a full superleaf height
+            // (the mixed method's second-panel Grams come from them)
+            c.leaf_rows = super_tiles<T>(32) * 64 * rows_per_lane<T>(32);
             c.leaves = (m[i] + c.leaf_rows - 1) / c.leaf_rows;
             c.vec = p.vec;
             c.ws = (int64_t)off;

@@ -29,19 +29,23 @@
 //   batch_mixed_gram (mixed) BF16x3 superleaf Grams, then
 //   batch_mixed_flat one wave per matrix sums them and runs the flat
 //                    Cholesky + gate
-//   batch_super_r    Householder superleaves (every superleaf of a plain
-//                    problem; of a mixed problem only if its gate failed)
+//   batch_leaf_r     (plain) Householder leaves of one tile, one wave each
+//                    (leaf1.hpp), reflectors stored over A
+//   batch_super_r    (mixed) Householder superleaves of the mixed problems
+//                    whose gate failed
 //   batch_tree_up    the ticketed walk up each problem's tree
 //   batch_down       one launch per tree level
 //   batch_mixed_q    (mixed) Q = A R^{-1} with the fused residual Gram, then
 //   batch_mixed_check  one wave per matrix sums it and runs the check; the
 //                    pair runs again for the second pass where a check asked
 //                    for it
-//   batch_super_q    Householder Q for the Householder problems
+//   batch_leaf_q     (plain) Q of every tile from its stored reflectors
+//   batch_super_q    (mixed) Householder Q for the problems whose gate failed
 #pragma once
 #include <type_traits>
 
 #include "gemm_mfma.hpp"
+#include "leaf1.hpp"
 #include "mixed.hpp"
 #include "super.hpp"
 
@@ -277,6 +281,44 @@ __global__ __launch_bounds__(batch_super_threads<T>()) void batch_super_q(const 
     } else {
         super_q_item<T>(probs, items[blockIdx.x], work, smem);
     }
+}
+
+// Plain problems: leaves of one tile (one item per tile), one wave each.
+template <typename T>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void batch_leaf_r(
+    const BatchProb* __restrict__ probs, const int2* __restrict__ items, char* __restrict__ work) {
+    __shared__ __align__(16) char smem[sizeof(TileLds<T, 32>)];
+    const int2 it = items[blockIdx.x];
+    const BatchProb& p = probs[it.x];
+    T* blocks = reinterpret_cast<T*>(prob_ws(p, work) + p.w.blocks);
+    dispatch_nv(p.N, p.vec, [&](auto NC, auto VC) {
+        constexpr int N = decltype(NC)::value;
+        constexpr bool VEC = decltype(VC)::value;
+        auto& L = *reinterpret_cast<TileLds<T, N>*>(smem);
+        if (p.d.lv[0].tiles == 1) {  // one tile: R to the caller, Q over A
+            tsqr_single_body<T, N, rows_per_lane<T>(N), VEC>(p.d, reinterpret_cast<T*>(p.A), p.lda,
+                                                              reinterpret_cast<T*>(p.R), p.ldr, L);
+            return;
+        }
+        leaf_r_tile<T, N, rows_per_lane<T>(N), VEC, true>(p.d, reinterpret_cast<T*>(p.A), p.lda, blocks, L, it.y);
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void batch_leaf_q(
+    const BatchProb* __restrict__ probs, const int2* __restrict__ items, char* __restrict__ work) {
+    __shared__ __align__(16) char smem[sizeof(TileLds<T, 32>)];
+    const int2 it = items[blockIdx.x];
+    const BatchProb& p = probs[it.x];
+    if (p.d.lv[0].tiles == 1) return;  // done by batch_leaf_r
+    const T* blocks = reinterpret_cast<const T*>(prob_ws(p, work) + p.w.blocks);
+    dispatch_nv(p.N, p.vec, [&](auto NC, auto VC) {
+        constexpr int N = decltype(NC)::value;
+        constexpr bool VEC = decltype(VC)::value;
+        auto& L = *reinterpret_cast<TileLds<T, N>*>(smem);
+        leaf_q_tile<T, N, rows_per_lane<T>(N), VEC>(p.d, reinterpret_cast<T*>(p.A), p.lda,
+                                                    leaf_s_block<T>(p.d, blocks, it.y), p.d.H, L, it.y);
+    });
 }
 
 // ---------------------------------------------------------------------------

@@ -29,8 +29,7 @@ void launch_panel_stage(const BatchDev& b, const BatchStage& s, char* work, hipS
         hipLaunchKernelGGL((batch_super_r<T, true>), dim3(hh_grid), dim3(batch_super_threads<T>()), 0, st, b.probs,
                            hh_items, hh_count, work);
     else
-        hipLaunchKernelGGL((batch_super_r<T, false>), dim3(hh_grid), dim3(batch_super_threads<T>()), 0, st, b.probs,
-                           hh_items, hh_count, work);
+        hipLaunchKernelGGL(batch_leaf_r<T>, dim3(s.leaves.count), dim3(64), 0, st, b.probs, s.leaves.items, work);
     if (s.level1.count)
         hipLaunchKernelGGL(batch_tree_up<T>, dim3(s.level1.count), dim3(64), 0, st, b.probs, s.level1.items, work);
     for (int l = (int)s.down.size() - 1; l >= 1; --l)
@@ -50,8 +49,7 @@ void launch_panel_stage(const BatchDev& b, const BatchStage& s, char* work, hipS
         hipLaunchKernelGGL((batch_super_q<T, true>), dim3(hh_grid), dim3(batch_super_threads<T>()), 0, st, b.probs,
                            hh_items, hh_count, work);
     else
-        hipLaunchKernelGGL((batch_super_q<T, false>), dim3(hh_grid), dim3(batch_super_threads<T>()), 0, st, b.probs,
-                           hh_items, hh_count, work);
+        hipLaunchKernelGGL(batch_leaf_q<T>, dim3(s.leaves.count), dim3(64), 0, st, b.probs, s.leaves.items, work);
 }
 }  // namespace
 

@@ -321,10 +321,9 @@ __device__ __forceinline__ T* path_s(const TreeDesc& d, const T* __restrict__ bl
 // A single tile (m <= H): factor, R to the caller, Q over A (unless form_q
 // is 0).
 template <typename T, int N, int RPL, bool VEC>
-__global__ __launch_bounds__(64) void tsqr_single(const TreeDesc d, T* __restrict__ A, int lda, T* __restrict__ R,
-                                                   int ldr, int form_q) {
-    __shared__ TileLds<T, N> L;
-    const int lane = threadIdx.x;
+__device__ __forceinline__ void tsqr_single_body(const TreeDesc& d, T* __restrict__ A, int lda, T* __restrict__ R,
+                                                 int ldr, TileLds<T, N>& L, bool form_q = true) {
+    const int lane = threadIdx.x & 63;
     const int n = d.n;
     const int rows = (int)d.lv[0].rows;
     T x[RPL][N];
@@ -334,6 +333,13 @@ __global__ __launch_bounds__(64) void tsqr_single(const TreeDesc d, T* __restric
     if (!form_q) return;  // R only: A stays as it was
     form_q_tile<T, N, RPL>(x, L, lane);
     store_rows<T, N, RPL, VEC>(x, A, lda, rows, n, lane);
+}
+
+template <typename T, int N, int RPL, bool VEC>
+__global__ __launch_bounds__(64) void tsqr_single(const TreeDesc d, T* __restrict__ A, int lda, T* __restrict__ R,
+                                                   int ldr, int form_q) {
+    __shared__ TileLds<T, N> L;
+    tsqr_single_body<T, N, RPL, VEC>(d, A, lda, R, ldr, L, form_q != 0);
 }
 
 // Down sweep of one tile of `level` (1 <= level <= root - 1): its explicit
