@@ -415,7 +415,11 @@ __global__ __launch_bounds__(64) void tsqr_mixed_q(const TreeDesc d, T* __restri
                                                    double* __restrict__ side, const int* __restrict__ ctl, int pass) {
     __shared__ double sums[12 * 64];
     if (!ctl[0] || (pass == 2 && !ctl[1])) return;
-    mixed_q_body<T, NT, VEC>(d.lv[0], d.n, A, lda, side + (int64_t)d.side_leaves * 1024,
+    // one workgroup per side superleaf (side_rows rows)
+    LevelDesc view = d.lv[0];
+    view.tile_rows = d.side_rows;
+    view.tiles = d.side_leaves;
+    mixed_q_body<T, NT, VEC>(view, d.n, A, lda, side + (int64_t)d.side_leaves * 1024,
                              side + (int64_t)blockIdx.x * 1024, sums, blockIdx.x);
 }
 

@@ -39,10 +39,12 @@ struct TreeDesc {
     int arity;   // children per parent
     int levels;  // number of levels (root is levels - 1)
     int total_ctr;
-    // FP64 32 x 32 side blocks kept per level-0 superleaf (the superleaf
-    // Gram paths); 0 for plans whose leaves are single Householder tiles
-    // (tile_leaf_plan), which need only the whole-matrix blocks after them
-    int side_leaves;
+    // FP64 32 x 32 side blocks kept per superleaf of side_rows rows (the
+    // superleaf Gram paths and the mixed Q pass), then the whole-matrix
+    // blocks.  Superleaf plans: the level-0 tiles themselves.  Plans with
+    // single-tile leaves: none (AUTO, HOUSEHOLDER), or superleaves of
+    // side_rows rows that group several leaves (CHOLQR2's Q pass).
+    int side_leaves, side_rows;
     int64_t block_elems;  // elements of all blocks
     LevelDesc lv[kMaxLevels];
 };
@@ -91,6 +93,7 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d, int arity
     d->levels = l + 1;
     d->total_ctr = ctr;
     d->side_leaves = d->lv[0].tiles;
+    d->side_rows = leaf_rows;
     d->block_elems = blk;
     return true;
 }

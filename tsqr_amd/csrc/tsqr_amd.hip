@@ -139,12 +139,16 @@ int check_args(int m, int n, const T* A, int lda, const T* R, int ldr) {
 template <typename T>
 size_t tsqr_ws_bytes(int m, int n) {
     if (n < 1 || m < n || n > 32) return 0;
-    // enough for either plan (single-tile leaves or superleaves), so the
-    // size does not depend on the path a call takes
-    TreeDesc d, e;
-    if (!plan_tsqr<T>(m, n, &d, false) || !plan_tsqr<T>(m, n, &e, true)) return 0;
-    const size_t a = tree_workspace_bytes(d, sizeof(T)), b = tree_workspace_bytes(e, sizeof(T));
-    return a > b ? a : b;
+    // enough for the plan of every leaf path, so that the size does not
+    // depend on the path a call takes
+    size_t most = 0;
+    for (int path = kLeafAuto; path <= kLeafCholQR2; ++path) {
+        TreeDesc d;
+        if (!plan_tsqr<T>(m, n, &d, path)) return 0;
+        const size_t b = tree_workspace_bytes(d, sizeof(T));
+        most = b > most ? b : most;
+    }
+    return most;
 }
 
 // Leaf path per dtype (tsqr_set_leaf_path).  FP32 defaults to the Gram path
@@ -178,7 +182,7 @@ int tsqr_core(int m, int n, T* A, int lda, T* R, int ldr, void* work, hipStream_
               bool r_only = false, bool untimed = false) {
     LaunchOpts o = mixed ? mixed_opts() : current_opts<T>();
     TreeDesc d;
-    if (!plan_tsqr<T>(m, n, &d, !mixed && tile_leaf_path(o.path))) return TSQR_ERR_SHAPE;
+    if (!plan_tsqr<T>(m, n, &d, o.path)) return TSQR_ERR_SHAPE;
     o.r_only = r_only ? 1 : 0;
     o.untimed = untimed ? 1 : 0;
     return tsqr_launch<T>(d, A, lda, R, ldr, work, o, st);
@@ -833,8 +837,8 @@ int tsqr_leaf_stats(const void* work, int m, int n, int dtype, int* gram_leaves,
     if (n < 1 || m < n || n > 32) return TSQR_ERR_SHAPE;
     TreeDesc d;
     const size_t elem = dtype == TSQR_FLOAT ? sizeof(float) : sizeof(double);
-    const bool tiles = tile_leaf_path(dtype == TSQR_FLOAT ? current_opts<float>().path : current_opts<double>().path);
-    const bool ok = dtype == TSQR_FLOAT ? plan_tsqr<float>(m, n, &d, tiles) : plan_tsqr<double>(m, n, &d, tiles);
+    const int path = dtype == TSQR_FLOAT ? current_opts<float>().path : current_opts<double>().path;
+    const bool ok = dtype == TSQR_FLOAT ? plan_tsqr<float>(m, n, &d, path) : plan_tsqr<double>(m, n, &d, path);
     if (!ok) return TSQR_ERR_SHAPE;
     *gram_leaves = 0;
     *householder_leaves = 0;
@@ -871,7 +875,7 @@ int tsqr_mixed_stats(const void* work, int m, int n, int* flat, int* passes, int
     if (!work || !flat || !passes || !check || !residual) return TSQR_ERR_ARG;
     if (n < 1 || m < n || n > 32) return TSQR_ERR_SHAPE;
     TreeDesc d;
-    if (!plan_tsqr<float>(m, n, &d)) return TSQR_ERR_SHAPE;
+    if (!plan_tsqr<float>(m, n, &d, mixed_opts().path)) return TSQR_ERR_SHAPE;
     *flat = 0;
     *passes = 0;
     *check = 0;
@@ -922,9 +926,9 @@ int tsqr_tree_arity(int dtype, int m, int n) {
     if (n < 1 || n > 32 || m < n) return TSQR_ERR_ARG;
     TreeDesc d;
     if (dtype == TSQR_FLOAT)
-        return plan_tsqr<float>(m, n, &d, tile_leaf_path(current_opts<float>().path)) ? d.arity : TSQR_ERR_SHAPE;
+        return plan_tsqr<float>(m, n, &d, current_opts<float>().path) ? d.arity : TSQR_ERR_SHAPE;
     if (dtype == TSQR_DOUBLE)
-        return plan_tsqr<double>(m, n, &d, tile_leaf_path(current_opts<double>().path)) ? d.arity : TSQR_ERR_SHAPE;
+        return plan_tsqr<double>(m, n, &d, current_opts<double>().path) ? d.arity : TSQR_ERR_SHAPE;
     return TSQR_ERR_ARG;
 }
 

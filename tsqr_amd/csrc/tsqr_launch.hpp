@@ -56,16 +56,27 @@ inline int leaf_rows_for(int m, int n) {
     return (int)tiles * H;
 }
 
-// tile_leaves: level-0 leaves of one tile each and no per-leaf side blocks
-// (the paths whose leaves are Householder tiles only, tile_leaf_path);
-// otherwise superleaves of leaf_rows_for rows for the superleaf Gram paths.
+// Plans the tree of a call on leaf path `path` (LeafPath):
+//   GRAM_TREE    superleaves of leaf_rows_for rows, each with a side block
+//                (its Gram matrix, then R^{-1});
+//   AUTO, HOUSEHOLDER  leaves of one tile (the Householder kernels of
+//                leaf1.hpp), no per-leaf side blocks;
+//   CHOLQR2      leaves of one tile, and side blocks for superleaves of
+//                leaf_rows_for rows (the mixed Q pass sums its Gram per
+//                superleaf).
 template <typename T>
-inline bool plan_tsqr(int m, int n, TreeDesc* d, bool tile_leaves = false) {
+inline bool plan_tsqr(int m, int n, TreeDesc* d, int path) {
     const int N = padded_cols(n);
     const int H = 64 * rows_per_lane<T>(N);
-    if (!make_tree(m, n, H, tile_leaves ? H : leaf_rows_for<T>(m, n), d, tuned_tree_arity(sizeof(T) == 4 ? 0 : 1, N)))
-        return false;
-    if (tile_leaves) d->side_leaves = 0;
+    const int sl = leaf_rows_for<T>(m, n);
+    const bool tiles = path != 2;  // kLeafGramTree (LeafPath, below)
+    if (!make_tree(m, n, H, tiles ? H : sl, d, tuned_tree_arity(sizeof(T) == 4 ? 0 : 1, N))) return false;
+    if (path == 0 || path == 1) {  // kLeafAuto, kLeafHouseholder
+        d->side_leaves = 0;
+    } else if (path == 3) {  // kLeafCholQR2
+        d->side_rows = sl;
+        d->side_leaves = (int)(((int64_t)m + sl - 1) / sl);
+    }
     return true;
 }
 
@@ -79,10 +90,8 @@ enum LeafPath {
                            // FP64 MFMA Gram for FP64; Householder tree if the gate rejects
 };
 
-// Paths planned with single-tile leaves: their level-0 work per tile is the
-// Householder kernels of leaf1.hpp (AUTO's fallback, HOUSEHOLDER); the flat
-// Gram passes of AUTO do not depend on the leaves.
-inline bool tile_leaf_path(int path) { return path == kLeafAuto || path == kLeafHouseholder; }
+static_assert(kLeafAuto == 0 && kLeafHouseholder == 1 && kLeafGramTree == 2 && kLeafCholQR2 == 3,
+              "plan_tsqr uses these values");
 
 struct LaunchOpts {
     int path;        // LeafPath

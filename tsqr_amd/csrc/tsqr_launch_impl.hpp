@@ -198,13 +198,15 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     // nearly never runs, and the superleaf R kernel walks up the tree itself
     // (tsqr_super_r_up).
     const bool flat_first = flat_auto || mixed;
-    // single-tile leaves: one wave per tile, the R kernel always walks up
-    const bool tile_leaves = d.side_leaves == 0 && d.lv[0].tile_rows == d.H;
+    // single-tile leaves: one wave per tile (leaf1.hpp)
+    const bool tile_leaves = d.lv[0].tile_rows == d.H;
     const int tile_grid = flat_first ? (leaves < kFallbackTileGrid ? leaves : kFallbackTileGrid) : leaves;
     // The fused walk saves a launch, which matters on short calls; on tall
     // matrices it measured slower (one wave walks while the rest of its
     // workgroup idles), and they keep the separate tree launch.
-    const bool fused_up = (tile_leaves || flat_first) && d.levels >= 2 && leaves <= kFusedWalkLeaves;
+    // the walk up needs every leaf to take its parent's ticket, so not on the
+    // Gram-tree path (its Gram leaves write their R in gram_chol)
+    const bool fused_up = (flat_first || (tile_leaves && !gram_tree)) && d.levels >= 2 && leaves <= kFusedWalkLeaves;
     if (tile_leaves) {
         // the reflectors go over A unless A must stay as it is (R only)
         auto leaf_r = [&](auto walk) {
@@ -258,10 +260,10 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     if (timed) timing_mark(kPhaseLeafQ, true, st);
     if (mixed) {
         for (int pass = 1; pass <= 2; ++pass) {
-            hipLaunchKernelGGL((tsqr_mixed_q<T, NT, VEC>), dim3(leaves), dim3(64), 0, st, d, A, lda, side,
+            hipLaunchKernelGGL((tsqr_mixed_q<T, NT, VEC>), dim3(d.side_leaves), dim3(64), 0, st, d, A, lda, side,
                                (const int*)ctl, pass);
             int np = 0;
-            const double* parts = reduce_grams(leaves, side, partial, ctl + (pass == 1 ? 0 : 1), &np, st);
+            const double* parts = reduce_grams(d.side_leaves, side, partial, ctl + (pass == 1 ? 0 : 1), &np, st);
             hipLaunchKernelGGL(tsqr_mixed_check<T>, dim3(1), dim3(kSumThreads), 0, st, d, np, parts, R, ldr, side, ctl, pass,
                                o.tol);
         }
