@@ -123,7 +123,7 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
     // item lists (problem indices into `all`)
     struct Stage {
         std::vector<int2> probs, leaves, level1;
-        std::vector<std::vector<int2>> down;
+        std::vector<std::vector<int2>> down, up;
     } st[2];
     for (int k = 0; k < (int)all.size(); ++k) {
         Stage& s = st[k < (int)first.size() ? 0 : 1];
@@ -135,6 +135,10 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
         for (int l = 1; l <= d.levels - 2; ++l) {
             if ((int)s.down.size() <= l) s.down.resize(l + 1);
             for (int t = 0; t < d.lv[l].tiles; ++t) s.down[l].push_back(make_int2(k, t));
+        }
+        for (int l = 1; l <= d.levels - 1; ++l) {
+            if ((int)s.up.size() <= l) s.up.resize(l + 1);
+            for (int t = 0; t < d.lv[l].tiles; ++t) s.up[l].push_back(make_int2(k, t));
         }
     }
     std::vector<int2> crow, clist;
@@ -149,6 +153,7 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
     for (auto& s : st) {
         nitems += s.probs.size() + s.leaves.size() + s.level1.size();
         for (auto& d : s.down) nitems += d.size();
+        for (auto& u : s.up) nitems += u.size();
     }
     bytes += nitems * sizeof(int2) + 256;
     std::vector<char> host(bytes, 0);
@@ -171,12 +176,13 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
     At a_crow = put_items(crow);
     At a_clist = put_items(clist);
     At a_probs[2], a_leaves[2], a_l1[2];
-    std::vector<At> a_down[2];
+    std::vector<At> a_down[2], a_up[2];
     for (int s = 0; s < 2; ++s) {
         a_probs[s] = put_items(st[s].probs);
         a_leaves[s] = put_items(st[s].leaves);
         a_l1[s] = put_items(st[s].level1);
         for (auto& d : st[s].down) a_down[s].push_back(put_items(d));
+        for (auto& u : st[s].up) a_up[s].push_back(put_items(u));
     }
     void* dmem = nullptr;
     if (hipMalloc(&dmem, bytes) != hipSuccess) return TSQR_ERR_ALLOC;
@@ -201,6 +207,8 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
         dev.stage[s].level1 = list(a_l1[s]);
         dev.stage[s].down.clear();
         for (auto& a : a_down[s]) dev.stage[s].down.push_back(list(a));
+        dev.stage[s].up.clear();
+        for (auto& a : a_up[s]) dev.stage[s].up.push_back(list(a));
     }
     b->dmem = dmem;
     b->ws_bytes = off ? off : 256;

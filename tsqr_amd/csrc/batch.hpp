@@ -33,7 +33,8 @@
 //                    (leaf1.hpp), reflectors stored over A
 //   batch_super_r    (mixed) Householder superleaves of the mixed problems
 //                    whose gate failed
-//   batch_tree_up    the ticketed walk up each problem's tree
+//   batch_tree_up    (mixed) the ticketed walk up each problem's tree
+//   batch_tree_level (plain) one tree level per launch
 //   batch_down       one launch per tree level
 //   batch_mixed_q    (mixed) Q = A R^{-1} with the fused residual Gram, then
 //   batch_mixed_check  one wave per matrix sums it and runs the check; the
@@ -338,6 +339,22 @@ __global__ __launch_bounds__(64) void batch_tree_up(const BatchProb* __restrict_
         constexpr int N = decltype(NC)::value;
         auto& L = *reinterpret_cast<TileLds<T, N>*>(smem);
         tree_up_body<T, N, rows_per_lane<T>(N)>(p.d, reinterpret_cast<T*>(p.R), p.ldr, blocks, tickets, L, it.y);
+    });
+}
+
+// Plain problems: one tree level per launch over every problem deep enough
+// (no tickets: an agent-scope release per block is an L2 write-back).
+template <typename T>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void batch_tree_level(
+    const BatchProb* __restrict__ probs, const int2* __restrict__ items, int level, char* __restrict__ work) {
+    __shared__ __align__(16) char smem[sizeof(TileLds<T, 32>)];
+    const int2 it = items[blockIdx.x];
+    const BatchProb& p = probs[it.x];
+    T* blocks = reinterpret_cast<T*>(prob_ws(p, work) + p.w.blocks);
+    dispatch_nv(p.N, 1, [&](auto NC, auto) {
+        constexpr int N = decltype(NC)::value;
+        auto& L = *reinterpret_cast<TileLds<T, N>*>(smem);
+        tree_level_body<T, N, rows_per_lane<T>(N)>(p.d, level, reinterpret_cast<T*>(p.R), p.ldr, blocks, L, it.y);
     });
 }
 

@@ -25,6 +25,7 @@ struct BatchStage {
     ItemList leaves;             // every superleaf
     ItemList level1;             // every level-1 tree block
     std::vector<ItemList> down;  // down[l]: every tile of level l (l >= 1)
+    std::vector<ItemList> up;    // up[l]: every tile of level l >= 1, root included
     int64_t fallback = 0;        // workspace byte offset of the mixed method's Householder list
 };
 

@@ -30,8 +30,15 @@ void launch_panel_stage(const BatchDev& b, const BatchStage& s, char* work, hipS
                            hh_items, hh_count, work);
     else
         hipLaunchKernelGGL(batch_leaf_r<T>, dim3(s.leaves.count), dim3(64), 0, st, b.probs, s.leaves.items, work);
-    if (s.level1.count)
-        hipLaunchKernelGGL(batch_tree_up<T>, dim3(s.level1.count), dim3(64), 0, st, b.probs, s.level1.items, work);
+    if (mixed) {
+        if (s.level1.count)
+            hipLaunchKernelGGL(batch_tree_up<T>, dim3(s.level1.count), dim3(64), 0, st, b.probs, s.level1.items, work);
+    } else {
+        for (int l = 1; l < (int)s.up.size(); ++l)
+            if (s.up[l].count)
+                hipLaunchKernelGGL(batch_tree_level<T>, dim3(s.up[l].count), dim3(64), 0, st, b.probs, s.up[l].items,
+                                   l, work);
+    }
     for (int l = (int)s.down.size() - 1; l >= 1; --l)
         if (s.down[l].count)
             hipLaunchKernelGGL(batch_down<T>, dim3(s.down[l].count), dim3(64), 0, st, b.probs, s.down[l].items, l,

@@ -234,14 +234,11 @@ __global__ __launch_bounds__(64) void tsqr_tree_up(const TreeDesc d, T* __restri
 // agent-scope release per block is needed (tree_up_body needs one per
 // block: an L2 write-back on MI355X).
 template <typename T, int N, int RPL, bool FORMQ = false>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void tsqr_tree_level(const TreeDesc d, int level, T* __restrict__ R, int ldr,
-                                                       T* __restrict__ blocks, const int* __restrict__ skip) {
-    __shared__ TileLds<T, N> L;
-    if (skip[0]) return;
-    const int lane = threadIdx.x;
+__device__ __forceinline__ void tree_level_body(const TreeDesc& d, int level, T* __restrict__ R, int ldr,
+                                                T* __restrict__ blocks, TileLds<T, N>& L, int tile) {
+    const int lane = threadIdx.x & 63;
     const int n = d.n;
     const LevelDesc& Lv = d.lv[level];
-    const int tile = blockIdx.x;
     const int rows = tile_rows_of(Lv, tile);
     T* blk = blocks + Lv.blk_off + (int64_t)tile * d.H * n;
     T x[RPL][N];
@@ -259,6 +256,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ts
         if constexpr (FORMQ) form_q_tile<T, N, RPL>(x, L, lane);
     }
     store_rows<T, N, RPL, true>(x, blk, d.H, rows, n, lane);
+}
+
+template <typename T, int N, int RPL, bool FORMQ = false>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void tsqr_tree_level(
+    const TreeDesc d, int level, T* __restrict__ R, int ldr, T* __restrict__ blocks, const int* __restrict__ skip) {
+    __shared__ TileLds<T, N> L;
+    if (skip[0]) return;
+    tree_level_body<T, N, RPL, FORMQ>(d, level, R, ldr, blocks, L, blockIdx.x);
 }
 
 // S of superleaf `leaf` after a FORMQ up sweep: B_1 B_2 ... B_top, B_l the
