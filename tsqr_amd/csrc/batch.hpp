@@ -393,8 +393,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ba
     const int n = p.d.n;
     dispatch_nv(p.N, p.vec, [&](auto NC, auto VC) {
         constexpr int NT = decltype(NC)::value > 16 ? 2 : 1;
-        mixed_gram_body<NT, decltype(VC)::value>(L0, n, reinterpret_cast<const float*>(p.A), p.lda,
-                                                 side + (int64_t)it.y * 1024, sums, it.y);
+        mixed_gram_body<NT, decltype(VC)::value>(p.d.m, n, reinterpret_cast<const float*>(p.A), p.lda,
+                                                 side + (int64_t)it.y * 1024, sums, it.y, L0.tiles);
     });
 }
 
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(256) void batch_mixed_flat(const BatchProb* __restr
     if (threadIdx.x >= 64) return;
     const int64_t T0 = L0.tiles;
     const bool ok = flat_factor<float>(L, p.d.n, reinterpret_cast<float*>(p.R), p.ldr, side + T0 * 1024, ctl, gate,
-                                       side + (T0 + 1) * 1024, lane);
+                                       side + (T0 + 1) * 1024, lane, true);
     if (!ok) {
         int at = 0;
         if (lane == 0) at = __hip_atomic_fetch_add(fallback, L0.tiles, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -436,6 +436,7 @@ __global__ __launch_bounds__(64, 2) void batch_mixed_q(const BatchProb* __restri
                                                      const int2* __restrict__ items, char* __restrict__ work,
                                                      int pass) {
     __shared__ double sums[12 * 64];
+    __shared__ float tr[32 * kTrPitch];
     const int2 it = items[blockIdx.x];
     const BatchProb& p = probs[it.x];
     char* ws = prob_ws(p, work);
@@ -444,10 +445,11 @@ __global__ __launch_bounds__(64, 2) void batch_mixed_q(const BatchProb* __restri
     double* side = prob_side(p, work);
     const LevelDesc& L0 = p.d.lv[0];
     const int64_t T0 = L0.tiles;
+    const bool bf16q = ctl[pass == 1 ? kCtlBf16Q1 : kCtlBf16Q2] != 0;
     dispatch_nv(p.N, p.vec, [&](auto NC, auto VC) {
         constexpr int NT = decltype(NC)::value > 16 ? 2 : 1;
-        mixed_q_body<float, NT, decltype(VC)::value>(L0, p.d.n, reinterpret_cast<float*>(p.A), p.lda,
-                                                     side + T0 * 1024, side + (int64_t)it.y * 1024, sums, it.y);
+        mixed_q_leaf<NT, decltype(VC)::value>(p.d.m, p.d.n, reinterpret_cast<float*>(p.A), p.lda, side + T0 * 1024,
+                                              side + (int64_t)it.y * 1024, sums, tr, it.y, (int)T0, bf16q);
     });
 }
 
@@ -479,20 +481,23 @@ __global__ __launch_bounds__(256) void batch_mixed_check(const BatchProb* __rest
 // second product to R12.
 // ---------------------------------------------------------------------------
 
-// P_b = Q1_b^T A2_b for row block `blk`, written to out (FP64 32 x 32,
-// row-major).  Lane l loads rows 4(l/16)..+3 of columns l%16 and 16 + l%16
-// of both matrices; batches of D 16-row slabs are loaded together and the
-// next batch is in flight while the current one is multiplied.
+// P_b = Q1_b^T A2_b for row block `blk` of q.leaves, written to out (FP64
+// 32 x 32, row-major), and the block's squared column norms of A2 to norms.
+// The row blocks partition the 64-row chunks by stride (block j: chunks j,
+// j + S, ..., S = q.leaves), so a couple's waves sweep it together; only
+// the sums over all blocks are used.  Lane l loads rows 4(l/16)..+3 of
+// columns l%16 and 16 + l%16 of both matrices in each of a chunk's four
+// 16-row slabs; the next chunk is in flight while the current one is
+// multiplied.
 template <typename T, bool VEC>
 __device__ __forceinline__ void cross_block(const BatchCouple& q, int blk, double* __restrict__ out,
                                             double* __restrict__ norms, int lane) {
     const int col = lane & 15;
     const int rq = 4 * (lane >> 4);
-    const int64_t r0 = (int64_t)blk * q.leaf_rows;
-    const int64_t left = q.m - r0;
-    const int rows = (int)(left < q.leaf_rows ? left : q.leaf_rows);
-    const T* Qb = reinterpret_cast<const T*>(q.Q1) + r0;
-    const T* Ab = reinterpret_cast<const T*>(q.A2) + r0;
+    const int S = q.leaves;
+    const int chunks = (q.m + 63) / 64;
+    const T* Q1 = reinterpret_cast<const T*>(q.Q1);
+    const T* A2 = reinterpret_cast<const T*>(q.A2);
     v4d c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
     double s0sq = 0.0, s1sq = 0.0;  // this lane's part of the squared column norms of A2
     auto mma4 = [&](const Raw4<T>& q0, const Raw4<T>& q1, const Raw4<T>& a0, const Raw4<T>& a1) {
@@ -508,49 +513,39 @@ __device__ __forceinline__ void cross_block(const BatchCouple& q, int blk, doubl
             c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y1, c11, 0, 0, 0);
         }
     };
-    constexpr int D = 4;  // 16-row slabs per batch
-    int s0 = 0;
-    if (VEC && rows >= D * 16) {
-        Raw4<T> cq0[D], cq1[D], ca0[D], ca1[D];
+    constexpr int D = 4;  // 16-row slabs per chunk
+    auto load = [&](int ch, Raw4<T>(&x0)[D], Raw4<T>(&x1)[D], Raw4<T>(&y0)[D], Raw4<T>(&y1)[D]) {
+        const int64_t r0 = (int64_t)ch * 64;
+        const int lim = (int)(q.m - r0 < 64 ? q.m - r0 : 64);
+        const T* Qb = Q1 + r0;
+        const T* Ab = A2 + r0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            cq0[k] = load_raw4<T, true>(Qb, q.lda, 16 * k + rq, col, rows, 32);
-            cq1[k] = load_raw4<T, true>(Qb, q.lda, 16 * k + rq, col + 16, rows, 32);
-            ca0[k] = load_raw4<T, true>(Ab, q.lda, 16 * k + rq, col, rows, q.n2);
-            ca1[k] = load_raw4<T, true>(Ab, q.lda, 16 * k + rq, col + 16, rows, q.n2);
+            x0[k] = load_raw4<T, VEC>(Qb, q.lda, 16 * k + rq, col, lim, 32);
+            x1[k] = load_raw4<T, VEC>(Qb, q.lda, 16 * k + rq, col + 16, lim, 32);
+            y0[k] = load_raw4<T, VEC>(Ab, q.lda, 16 * k + rq, col, lim, q.n2);
+            y1[k] = load_raw4<T, VEC>(Ab, q.lda, 16 * k + rq, col + 16, lim, q.n2);
         }
+    };
+    Raw4<T> cq0[D], cq1[D], ca0[D], ca1[D];
+    int ch = blk;
+    if (ch < chunks) load(ch, cq0, cq1, ca0, ca1);
 #pragma unroll 1
-        for (;;) {
-            const int next = s0 + D * 16;
-            const bool more = next + D * 16 <= rows;
-            Raw4<T> pq0[D], pq1[D], pa0[D], pa1[D];
-            if (more) {
+    for (; ch < chunks; ch += S) {
+        const int nx = ch + S;
+        Raw4<T> pq0[D], pq1[D], pa0[D], pa1[D];
+        if (nx < chunks) load(nx, pq0, pq1, pa0, pa1);
 #pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    pq0[k] = load_raw4<T, true>(Qb, q.lda, next + 16 * k + rq, col, rows, 32);
-                    pq1[k] = load_raw4<T, true>(Qb, q.lda, next + 16 * k + rq, col + 16, rows, 32);
-                    pa0[k] = load_raw4<T, true>(Ab, q.lda, next + 16 * k + rq, col, rows, q.n2);
-                    pa1[k] = load_raw4<T, true>(Ab, q.lda, next + 16 * k + rq, col + 16, rows, q.n2);
-                }
-            }
+        for (int k = 0; k < D; ++k) mma4(cq0[k], cq1[k], ca0[k], ca1[k]);
+        if (nx >= chunks) break;
 #pragma unroll
-            for (int k = 0; k < D; ++k) mma4(cq0[k], cq1[k], ca0[k], ca1[k]);
-            s0 = next;
-            if (!more) break;
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                cq0[k] = pq0[k];
-                cq1[k] = pq1[k];
-                ca0[k] = pa0[k];
-                ca1[k] = pa1[k];
-            }
+        for (int k = 0; k < D; ++k) {
+            cq0[k] = pq0[k];
+            cq1[k] = pq1[k];
+            ca0[k] = pa0[k];
+            ca1[k] = pa1[k];
         }
     }
-#pragma unroll 1
-    for (; s0 < rows; s0 += 16)
-        mma4(load_raw4<T, VEC>(Qb, q.lda, s0 + rq, col, rows, 32), load_raw4<T, VEC>(Qb, q.lda, s0 + rq, col + 16, rows, 32),
-             load_raw4<T, VEC>(Ab, q.lda, s0 + rq, col, rows, q.n2),
-             load_raw4<T, VEC>(Ab, q.lda, s0 + rq, col + 16, rows, q.n2));
     // f64 accumulator: lane l, register v holds C[4v + l/16][l%16]
     const int i0 = lane >> 4;
 #pragma unroll
@@ -635,33 +630,34 @@ __global__ __launch_bounds__(256) void batch_cross_sum(const BatchCouple* __rest
     }
 }
 
-// A2 rows of one block -= Q1 rows * P on the matrix cores in the working
-// precision (v_mfma_f32_16x16x4f32 for float, v_mfma_f64_16x16x4f64 for
-// double), accumulators initialised from A2 and B = -P, one rounding.
-// Rows in blocks of 64; lane l feeds row 4(l%16) + t of column 4 kg + l/16
+// A2 rows of row block `blk` (the chunks blk, blk + S, ... of cross_block)
+// -= Q1 rows * P on the matrix cores in the working precision
+// (v_mfma_f32_16x16x4f32 for float, v_mfma_f64_16x16x4f64 for double),
+// accumulators initialised from A2 and B = -P, one rounding.
+// Rows in chunks of 64; lane l feeds row 4(l%16) + t of column 4 kg + l/16
 // to MFMA t, so output register v of (t, jt) is row Mfma16<T>::row(v, l/16)
 // * 4 + t (16(l/16) + 4v + t for float, 16v + 4(l/16) + t for double),
 // column 16 jt + l%16, and each lane stores 4 consecutive rows per (v, jt).
-// The next block's loads are issued before the current block's MFMAs.
+// The next chunk's loads are issued before the current chunk's MFMAs.
 //
 // With GRAM (float, q.gram set) the new rows also go through the BF16x3
 // Gram products of the mixed method (GramAcc32, FP32 per 64 rows, then FP64
 // sums in `sums`) straight from the accumulators: lane (li, lq) holds rows
-// 16 lq .. 16 lq + 15 of columns li and 16 + li of each 64-row block and
+// 16 lq .. 16 lq + 15 of columns li and 16 + li of each 64-row chunk and
 // feeds rows 16 lq + 8 h + s to k-slot s of the h-th 32-row product, the
-// same assignment in every lane.  The result is the second panel's Gram
-// matrix of superleaf blk, written to G.
+// same assignment in every lane.  The result goes to G as the second
+// panel's Gram matrix of superleaf blk (the flat path only sums them).
 template <typename T, bool VEC, bool GRAM = false>
 __device__ __forceinline__ void project_block(const BatchCouple& q, int blk, const double* __restrict__ P, int lane,
                                               double* sums = nullptr, double* __restrict__ G = nullptr) {
     static_assert(!GRAM || sizeof(T) == 4, "the fused Gram is the mixed (FP32) method's");
     using M = Mfma16<T>;
     const int li = lane & 15, lq = lane >> 4;
-    const int64_t r0 = (int64_t)blk * q.leaf_rows;
-    const int64_t left = q.m - r0;
-    const int rows = (int)(left < q.leaf_rows ? left : q.leaf_rows);
-    const T* Qb = reinterpret_cast<const T*>(q.Q1) + r0;
-    T* Ab = reinterpret_cast<T*>(q.A2) + r0;
+    const int S = q.leaves;
+    const int chunks = (q.m + 63) / 64;
+    const T* Q1 = reinterpret_cast<const T*>(q.Q1);
+    T* A2 = reinterpret_cast<T*>(q.A2);
+    auto lim_of = [&](int ch) { return (int)(q.m - (int64_t)ch * 64 < 64 ? q.m - (int64_t)ch * 64 : 64); };
     T mb[8][2];  // -P in the B-operand layout of k-group kg (k = 4 kg + l/16)
 #pragma unroll
     for (int kg = 0; kg < 8; ++kg)
@@ -669,23 +665,26 @@ __device__ __forceinline__ void project_block(const BatchCouple& q, int blk, con
         for (int jt = 0; jt < 2; ++jt) mb[kg][jt] = (T)(-P[(4 * kg + lq) * 32 + 16 * jt + li]);
     auto row_of = [&](int v) { return 4 * M::row(v, lq); };  // first of the 4 rows of register v
     Four<T> qa[8], cv[2][4];
-    auto load_block = [&](int s0) {
+    auto load_block = [&](int ch) {
+        const T* Qb = Q1 + (int64_t)ch * 64;
+        const T* Ab = A2 + (int64_t)ch * 64;
+        const int lim = lim_of(ch);
 #pragma unroll
-        for (int kg = 0; kg < 8; ++kg) qa[kg] = load_four<T, VEC>(Qb, q.lda, s0 + 4 * li, 4 * kg + lq, rows, 32);
+        for (int kg = 0; kg < 8; ++kg) qa[kg] = load_four<T, VEC>(Qb, q.lda, 4 * li, 4 * kg + lq, lim, 32);
 #pragma unroll
         for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                cv[jt][v] = load_four<T, VEC>(Ab, q.lda, s0 + row_of(v), 16 * jt + li, rows, q.n2);
+                cv[jt][v] = load_four<T, VEC>(Ab, q.lda, row_of(v), 16 * jt + li, lim, q.n2);
     };
     GramAcc32 g;
     if constexpr (GRAM) {
         g.sums = sums;
         g.zero(lane);
     }
-    if (rows > 0) load_block(0);
+    if (blk < chunks) load_block(blk);
 #pragma unroll 1
-    for (int s0 = 0; s0 < rows; s0 += 64) {
+    for (int ch = blk; ch < chunks; ch += S) {
         typename M::v4 acc[4][2];
 #pragma unroll
         for (int jt = 0; jt < 2; ++jt)
@@ -696,7 +695,7 @@ __device__ __forceinline__ void project_block(const BatchCouple& q, int blk, con
         Four<T> x[8];
 #pragma unroll
         for (int kg = 0; kg < 8; ++kg) x[kg] = qa[kg];
-        if (s0 + 64 < rows) load_block(s0 + 64);
+        if (ch + S < chunks) load_block(ch + S);
 #pragma unroll
         for (int kg = 0; kg < 8; ++kg)
 #pragma unroll
@@ -710,7 +709,7 @@ __device__ __forceinline__ void project_block(const BatchCouple& q, int blk, con
                 Four<T> o;
 #pragma unroll
                 for (int t = 0; t < 4; ++t) o.v[t] = acc[t][jt][v];
-                store_four<T, VEC>(o, Ab, q.lda, s0 + row_of(v), 16 * jt + li, rows, q.n2);
+                store_four<T, VEC>(o, A2 + (int64_t)ch * 64, q.lda, row_of(v), 16 * jt + li, lim_of(ch), q.n2);
             }
         }
         if constexpr (GRAM) {

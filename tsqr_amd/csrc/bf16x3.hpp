@@ -38,6 +38,22 @@ __device__ __forceinline__ Split3 split3(const float (&x)[8]) {
     return s;
 }
 
+// The same split of FP64 values (the pieces carry the leading 24 bits; the
+// residuals are formed in FP64).
+__device__ __forceinline__ Split3 split3(const double (&x)[8]) {
+    Split3 s;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const __bf16 h = (__bf16)(float)x[t];
+        const double r = x[t] - (double)(float)h;
+        const __bf16 m = (__bf16)(float)r;
+        s.h[t] = h;
+        s.m[t] = m;
+        s.l[t] = (__bf16)(float)(r - (double)(float)m);
+    }
+    return s;
+}
+
 __device__ __forceinline__ f4v mma_bf16(const bf16x8& a, const bf16x8& b, f4v c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }

@@ -536,25 +536,6 @@ __device__ __forceinline__ void sum_blocks(const double* __restrict__ src, int c
     for (int e = 0; e < 4; ++e) out[e] = s[e];
 }
 
-// Flat Gram path, stage 1: partial[p] = sum of the 32 x 32 blocks
-// [p*c, (p+1)*c) of side, c = ceil(leaves / gridDim.x), added in order
-// (deterministic).  256 threads, 4 entries each, eight blocks in flight.
-// Does nothing if run_if is given and run_if[0] == 0.  (A template only so
-// that every translation unit that includes this header gets its own copy.)
-template <int UNUSED = 0>
-__global__ __launch_bounds__(256) void tsqr_gram_partial(int leaves, const double* __restrict__ side,
-                                                         double* __restrict__ partial,
-                                                         const int* __restrict__ run_if) {
-    if (run_if && !run_if[0]) return;
-    const int per = (leaves + gridDim.x - 1) / gridDim.x;
-    const int b0 = blockIdx.x * per;
-    const int b1 = b0 + per < leaves ? b0 + per : leaves;
-    double acc[4];
-    sum_blocks(side + (int64_t)b0 * 1024, b1 > b0 ? b1 - b0 : 0, threadIdx.x, acc);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) partial[(int64_t)blockIdx.x * 1024 + threadIdx.x + 256 * q] = acc[q];
-}
-
 // L.G = sum of the nparts 32 x 32 FP64 partials, added in index order
 // (kSumThreads = 256 threads, thread t owns entries t + 256 e).  The
 // partials were written by other CUs, so every load goes to memory: eight
@@ -624,12 +605,14 @@ __device__ __forceinline__ bool chol_inv_wave(GramLds& L, int n, int lane, doubl
 // Cholesky of the whole-matrix Gram matrix in L.G by one wave (lanes
 // 0..63).  If it succeeds and ||R||_F ||R^{-1}||_F <= gate: R (rounded to
 // T) goes to the caller, R^{-1} (FP64, row-major) to X, R itself in FP64
-// (row-major) to rkeep (if not null) and ctl[0] = 1.  Otherwise nothing is
-// written.  Returns whether the gate passed (the same in every lane).
+// (row-major) to rkeep (if not null) and ctl[0] = 1; with bf16q, also
+// ctl[kCtlBf16Q1] = (kappa_est <= 2n), the mixed path's test for a Q pass on
+// the BF16 matrix cores.  Otherwise nothing is written.  Returns whether the
+// gate passed (the same in every lane).
 template <typename T>
 __device__ __forceinline__ bool flat_factor(GramLds& L, int n, T* __restrict__ R, int ldr, double* __restrict__ X,
                                             int* __restrict__ ctl, double gate, double* __restrict__ rkeep,
-                                            int lane) {
+                                            int lane, bool bf16q = false) {
     const int c = lane & 31;
     const bool hi = lane >= 32;
     double g[32];
@@ -656,7 +639,10 @@ __device__ __forceinline__ bool flat_factor(GramLds& L, int n, T* __restrict__ R
     } else {
 #pragma unroll
         for (int r = 0; r < 32; ++r) X[c * 32 + r] = g[r];
-        if (c == 0) ctl[0] = 1;
+        if (c == 0) {
+            ctl[0] = 1;
+            if (bf16q) ctl[kCtlBf16Q1] = (kest <= 2.0 * n) ? 1 : 0;
+        }
     }
     return ok;
 }
@@ -669,13 +655,14 @@ template <typename T>
 __global__ __launch_bounds__(kSumThreads) void tsqr_gram_flat(const TreeDesc d, int nparts,
                                                         const double* __restrict__ partial, T* __restrict__ R,
                                                         int ldr, double* __restrict__ side, int* __restrict__ ctl,
-                                                        double gate, double* __restrict__ rkeep) {
+                                                        double gate, double* __restrict__ rkeep,
+                                                        int bf16q = 0) {
     __shared__ GramLds L;
     const int t = threadIdx.x;
     sum_partials(L, nparts, partial, t);
     __syncthreads();
     if (t >= 64) return;
-    flat_factor<T>(L, d.n, R, ldr, side + (int64_t)d.side_leaves * 1024, ctl, gate, rkeep, t);
+    flat_factor<T>(L, d.n, R, ldr, side + (int64_t)d.side_leaves * 1024, ctl, gate, rkeep, t, bf16q != 0);
 }
 
 // ---------------------------------------------------------------------------

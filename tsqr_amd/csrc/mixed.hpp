@@ -19,21 +19,24 @@
 //   pass 1  G1 = A^T A (BF16x3, above), streamed over the whole matrix
 //           (tsqr_mixed_stream), the 64-row partials summed in FP64;
 //           R1 = chol(G1) and kappa_est = ||R1||_F ||R1^{-1}||_F <= gate in
-//           FP64 (tsqr_gram_flat); Q1 = A R1^{-1} in FP64 MFMA, rounded once
-//           (tsqr_mixed_q), and in the same pass G2 = Q1^T Q1 of the rounded
-//           Q1 (BF16x3 again).
+//           FP64 (tsqr_gram_flat); Q1 = A R1^{-1} streamed (tsqr_mixed_qs):
+//           BF16x3 products summed in FP32 where kappa_est <= 2n (R1^{-1}
+//           well conditioned), else FP64 MFMA products rounded once; and in
+//           the same pass G2 = Q1^T Q1 of the stored Q1 (BF16x3 again).
 //   check   e1 = ||G2 - I||_F (the orthogonality of the stored Q1).  If
 //           e1 <= tol the result is Q1, R1 (tsqr_mixed_check).
-//   pass 2  otherwise R2 = chol(G2), Q = Q1 R2^{-1} (the same kernel, with
-//           G3 = Q^T Q fused again), R = R2 R1 in FP64; e2 = ||G3 - I||_F is
-//           recorded and status 1 is set if e2 > tol.
+//   pass 2  otherwise R2 = chol(G2), Q = Q1 R2^{-1} (the same kernel, on
+//           BF16x3 where e1 <= kBf16QMaxResidual, with G3 = Q^T Q fused
+//           again), R = R2 R1 in FP64; e2 = ||G3 - I||_F is recorded and
+//           status 1 is set if e2 > tol.
 // If the first Cholesky breaks down or kappa_est exceeds the gate, the whole
 // matrix takes the Householder tree instead (the superleaf and tree kernels,
 // driven by the same ctl word), so no input is rejected.
 //
 // ctl words (int): ctl[0] = 1 flat Gram taken, ctl[1] = 1 second pass
 // requested, ctl[2] = status (0 ok, 1 residual above tol after the second
-// pass, 2 second Cholesky broke down).  The FP64 side block T0+2 holds
+// pass, 2 second Cholesky broke down), ctl[3] / ctl[4] = 1 where pass 1 /
+// pass 2 forms Q on the BF16 matrix cores.  The FP64 side block T0+2 holds
 // e1, e2.
 #pragma once
 #include "bf16x3.hpp"
@@ -108,46 +111,46 @@ __device__ __forceinline__ void load8(float (&x)[8], const float* __restrict__ b
     }
 }
 
-// Pass 1a of the batch: G_b = A_b^T A_b of one superleaf (one wave) with
-// BF16x3 products, written to G (FP64, row-major 32 x 32).  Lane l loads
-// rows 8(l/16)..+7 of column l%16 (and 16 + l%16) of each 32-row step; the
-// two steps of a 64-row batch are loaded together and the next batch is in
+// Pass 1a of the batch: the BF16x3 Gram matrix of superleaf `blk` of S of
+// an m-row panel (one wave), written to G (FP64, row-major 32 x 32).  The
+// superleaves partition the panel's 64-row chunks by stride: superleaf j
+// takes chunks j, j + S, j + 2S, ..., so the S waves of a panel sweep it
+// together over a window of S chunks (the Grams are only ever summed).  Lane
+// l loads rows 8(l/16)..+7 of column l%16 (and 16 + l%16) of each 32-row
+// step; a chunk's two steps are loaded together and the next chunk is in
 // flight while the current one is multiplied.
 template <int NT, bool VEC>
-__device__ __forceinline__ void mixed_gram_body(const LevelDesc& L0, int n, const float* __restrict__ A, int lda,
-                                                double* __restrict__ G, double* sums, int leaf) {
+__device__ __forceinline__ void mixed_gram_body(int m, int n, const float* __restrict__ A, int lda,
+                                                double* __restrict__ G, double* sums, int blk, int S) {
     const int lane = threadIdx.x & 63;
-    const int rows = leaf_rows_of(L0, leaf);
-    const float* base = A + (int64_t)leaf * L0.tile_rows;
     const int col = lane & 15;
     const int rq = 8 * (lane >> 4);
-    constexpr int D = 2;  // 32-row steps per batch (a batch and its successor in flight: 64 VGPRs per column half)
+    const int chunks = (m + 63) / 64;
+    constexpr int D = 2;  // 32-row steps per chunk (a chunk and its successor in flight: 64 VGPRs per column half)
     GramAcc32 g;
     g.sums = sums;
     g.zero(lane);
-    float c0[D][8], c1[D][8];
-#pragma unroll
-    for (int q = 0; q < D; ++q) {
-        load8<VEC>(c0[q], base, lda, 32 * q + rq, col, rows, n);
-        if constexpr (NT == 2) load8<VEC>(c1[q], base, lda, 32 * q + rq, col + 16, rows, n);
-    }
-#pragma unroll 1
-    for (int s0 = 0; s0 < rows; s0 += 32 * D) {
-        const int next = s0 + 32 * D;
-        float p0[D][8], p1[D][8];
-        if (next < rows) {
-#pragma unroll
-            for (int q = 0; q < D; ++q) {
-                load8<VEC>(p0[q], base, lda, next + 32 * q + rq, col, rows, n);
-                if constexpr (NT == 2) load8<VEC>(p1[q], base, lda, next + 32 * q + rq, col + 16, rows, n);
-            }
-        }
+    auto load = [&](int ch, float(&x0)[D][8], float(&x1)[D][8]) {
+        const float* b = A + (int64_t)ch * 64;
+        const int lim = m - 64 * ch;
 #pragma unroll
         for (int q = 0; q < D; ++q) {
-            g.add<NT>(c0[q], c1[q]);
-            if (q & 1) g.flush<NT>(lane);
+            load8<VEC>(x0[q], b, lda, 32 * q + rq, col, lim, n);
+            if constexpr (NT == 2) load8<VEC>(x1[q], b, lda, 32 * q + rq, col + 16, lim, n);
         }
-        if (next >= rows) break;
+    };
+    float c0[D][8], c1[D][8];
+    int ch = blk;
+    if (ch < chunks) load(ch, c0, c1);
+#pragma unroll 1
+    for (; ch < chunks; ch += S) {
+        const int nx = ch + S;
+        float p0[D][8], p1[D][8];
+        if (nx < chunks) load(nx, p0, p1);
+#pragma unroll
+        for (int q = 0; q < D; ++q) g.add<NT>(c0[q], c1[q]);
+        g.flush<NT>(lane);
+        if (nx >= chunks) break;
 #pragma unroll
         for (int q = 0; q < D; ++q)
 #pragma unroll
@@ -157,6 +160,45 @@ __device__ __forceinline__ void mixed_gram_body(const LevelDesc& L0, int n, cons
             }
     }
     g.store(G, lane);
+}
+
+// End of a streaming launch that sums a Gram matrix with GramAcc32 (256
+// threads, P = gridDim.x workgroups): the four waves' FP64 slabs sums[wave]
+// are added in wave order into L.G (accumulator layout: lane l, register v
+// holds row 4(l/16) + v, column l%16 of a block), then stream_finish.
+template <int NT>
+__device__ __forceinline__ void gram32_finish(GramLds& L, const double (*sums)[12 * 64], double* __restrict__ partial,
+                                              int* __restrict__ tk, int* bcast) {
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wv = t >> 6;
+    const int col = lane & 15;
+    const int i0 = 4 * (lane >> 4);
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+        if (wv == s) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + v;
+                const double g00 = sums[s][v * 64 + lane];
+                const double g01 = NT == 2 ? sums[s][(4 + v) * 64 + lane] : 0.0;
+                const double g11 = NT == 2 ? sums[s][(8 + v) * 64 + lane] : 0.0;
+                if (s == 0) {
+                    L.G[i][col] = g00;
+                    L.G[i][16 + col] = g01;
+                    L.G[16 + col][i] = g01;
+                    L.G[16 + i][16 + col] = g11;
+                } else {
+                    L.G[i][col] += g00;
+                    L.G[i][16 + col] += g01;
+                    L.G[16 + col][i] += g01;
+                    L.G[16 + i][16 + col] += g11;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    stream_finish(L, partial, tk, bcast);
 }
 
 // Pass 1a of a single matrix, streamed: G = A^T A with the BF16x3 products
@@ -249,178 +291,375 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
         }
         g.flush<NT>(lane);
     }
-    // the workgroup's sum in L.G, waves added in order (accumulator layout:
-    // lane l, register v holds row 4(l/16) + v, column l%16 of a block)
-    const int i0 = 4 * (lane >> 4);
-#pragma unroll 1
-    for (int s = 0; s < 4; ++s) {
-        if (wv == s) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = i0 + v;
-                const double g00 = sums[s][v * 64 + lane];
-                const double g01 = NT == 2 ? sums[s][(4 + v) * 64 + lane] : 0.0;
-                const double g11 = NT == 2 ? sums[s][(8 + v) * 64 + lane] : 0.0;
-                if (s == 0) {
-                    L.G[i][col] = g00;
-                    L.G[i][16 + col] = g01;
-                    L.G[16 + col][i] = g01;
-                    L.G[16 + i][16 + col] = g11;
-                } else {
-                    L.G[i][col] += g00;
-                    L.G[i][16 + col] += g01;
-                    L.G[16 + col][i] += g01;
-                    L.G[16 + i][16 + col] += g11;
-                }
-            }
-        }
-        __syncthreads();
-    }
-    stream_finish(L, partial, tk, &bcast);
+    gram32_finish<NT>(L, sums, partial, tk, &bcast);
 }
 
-// Pass 1b / 2: Q = A X in place for every superleaf (X upper triangular,
-// FP64 row-major 32 x 32 in side block T0), products and sums in FP64 on the
-// matrix cores and one rounding, as in tsqr_gram_q's flat case; then the
-// Gram matrix of the rounded rows goes to side[leaf]: BF16x3 for float, FP64
-// MFMA for double.  Pass 1 runs when the flat Gram path was taken (ctl[0]);
-// pass 2 only if the check asked for it (ctl[1]).
+// ---------------------------------------------------------------------------
+// Q passes: Q = A X in place (X upper triangular, FP64 row-major 32 x 32),
+// and in the same pass the Gram matrix of the stored Q (BF16x3 for float,
+// FP64 MFMA for double).  Pass 1 runs when the flat Gram path was taken
+// (ctl[0]); pass 2 only if the check asked for it (ctl[1]).
 //
-// Layout: rows in blocks of 64; the FP64 MFMA output register v of (t, jt) is
-// row r0 + 16v + 4(l/16) + t, column 16 jt + l%16.
-//   float:  the 32 rows r0 + 32h + [0, 32) feed BF16 k-slots
-//           s = 4(v - 2h) + t of lane l, i.e. k = 8(l/16) + s <-> row
-//           16(v - 2h) + 4(l/16) + t: a bijection onto the 32 rows,
-//           identical for both operands;
-//   double: register (v, t) of lane l is the 16x16x4 f64 operand with
-//           k = l/16 <-> row 16v + 4(l/16) + t, again the same for both
-//           operands.
-template <typename T, int NT, bool VEC>
-__device__ __forceinline__ void mixed_q_body(const LevelDesc& L0, int n, T* __restrict__ A, int lda,
-                                             const double* __restrict__ X, double* __restrict__ Gout, double* sums,
-                                             int leaf) {
-    const int lane = threadIdx.x & 63;
+// Rows go in chunks of 16 RPL (RPL = q_rpl<T>(): 4 rows of FP32, 2 of
+// FP64, i.e. 16 bytes per lane and column).  A chunk is loaded as 16-byte
+// column runs: lane l holds rows RPL(l%16) .. +RPL-1 of column 4p + l/16 in
+// buf[p], p = 0..7 (zero past the rows and columns).  The products take one
+// of two forms:
+//
+// FP64 (every double pass; float where X is not known to be well
+// conditioned): 16x16x4 FP64 MFMA, element t of buf[kg] is the A operand
+// with k = l/16 <-> column 4kg + l/16 and i = l%16 <-> row RPL(l%16) + t;
+// products and sums in FP64, one rounding.  Output register v of (t, jt)
+// is Q[r0 + RPL(4v + l/16) + t][16 jt + l%16].
+//
+// BF16x3 (float, flag ctl[kCtlBf16Q1] / ctl[kCtlBf16Q2]): BF16x3 products
+// (bf16x3.hpp) summed in FP32 on v_mfma_f32_16x16x32_bf16 with k = the 32
+// columns of A.  Those sums carry an error of about u32 |A| |X| per entry,
+// i.e. u32 kappa(X) relative to Q, which is why the flag is set only for a
+// well-conditioned X (flat_factor, mixed_check_body).  The A operand needs
+// eight columns of one row per lane, so the chunk goes through the wave's
+// LDS tile `tw` (column c, row r at c * kTrPitch + r) and comes back as row
+// tile rt (16 rows): lane l takes A[r0 + 16 rt + l%16][8(l/16) + s], s =
+// 0..7.  X is the B operand, X[8(l/16) + s][16 jt + l%16] (zero outside
+// n x n).  Output register v of (rt, jt) is Q[r0 + 16 rt + 4(l/16) + v][16 jt
+// + l%16].
+//
+// For float, either form leaves lane l with q[jt][u][t] = Q[r0 + 16u + 4(l/16)
+// + t][16 jt + l%16]; the chunk goes back through the LDS tile so that it is
+// stored in the load layout (16-byte runs, 256 contiguous bytes per column
+// and instruction), and q feeds the Gram: the 32 rows r0 + 32h + [0, 32) are
+// BF16 k-slot s = 4(u - 2h) + t of lane l, a bijection onto the 32 rows,
+// identical for both operands.  For double, register (v, t) of lane l is
+// the 16x16x4 FP64 Gram operand with k = l/16 <-> row RPL(4v + l/16) + t.
+// Zero rows and columns of the operands make the padding of Q exactly zero,
+// and padded columns are not stored.
+// ---------------------------------------------------------------------------
+constexpr int kTrPitch = 66;  // floats per LDS column (8-byte aligned, reads conflict-free per half-wave)
+
+template <typename T>
+constexpr int q_rpl() {
+    return sizeof(T) == 4 ? 4 : 2;
+}
+
+// x[t] = rows r0 + t (t < RPL) of column c, zero outside rows and columns.
+template <typename T, int RPL, bool VEC>
+__device__ __forceinline__ void load_run(T (&x)[RPL], const T* __restrict__ base, int64_t ld, int r0, int c,
+                                         int rows, int ncols) {
+    using C = Chunk16<T, RPL>;
+    static_assert(C::kChunks == 1, "one 16-byte run per lane");
+    if (c < ncols) {
+        const T* p = base + (int64_t)c * ld + r0;
+        if (VEC && r0 + RPL <= rows) {
+            const typename C::vec v = *reinterpret_cast<const typename C::vec*>(p);
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) x[t] = v[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) x[t] = (r0 + t < rows) ? p[t] : T(0);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) x[t] = T(0);
+    }
+}
+
+template <typename T, bool VEC>
+__device__ __forceinline__ void q_chunk_load(T (&buf)[8][q_rpl<T>()], const T* __restrict__ base, int lda, int r0,
+                                             int rows, int n, int lane) {
+    constexpr int RPL = q_rpl<T>();
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+        load_run<T, RPL, VEC>(buf[p], base, lda, r0 + RPL * (lane & 15), 4 * p + (lane >> 4), rows, n);
+}
+
+// B operands of the FP64 form: mb[kg][jt] = X[4kg + l/16][16jt + l%16].
+__device__ __forceinline__ void q_x_f64(double (&mb)[8][2], const double* __restrict__ X, int lane) {
     const int li = lane & 15, lq = lane >> 4;
-    // B operand of k-group kg (k = 4 kg + l/16), column tile jt
-    double mb[8][2];
 #pragma unroll
     for (int kg = 0; kg < 8; ++kg)
 #pragma unroll
         for (int jt = 0; jt < 2; ++jt) mb[kg][jt] = X[(4 * kg + lq) * 32 + 16 * jt + li];
-    const int rows = leaf_rows_of(L0, leaf);
-    T* base = A + (int64_t)leaf * L0.tile_rows;
-    const int nk = (n + 3) / 4;
-    Raw4<T> raw[8];
+}
+
+// B operands of the BF16 form, split: X[8(l/16) + s][16jt + l%16].
+template <int NT>
+__device__ __forceinline__ void q_x_bf16(Split3 (&xb)[NT], const double* __restrict__ X, int n, int lane) {
+    const int li = lane & 15, lq = lane >> 4;
 #pragma unroll
-    for (int kg = 0; kg < 8; ++kg) {
-        const int k = 4 * kg + lq;
-        if (kg < nk) raw[kg] = load_raw4<T, VEC>(base, lda, 4 * li, k < n ? k : n - 1, rows, n);
-    }
-    GramAcc32 g;  // float: BF16x3, FP64 sums in LDS
-    v4d d00 = {0, 0, 0, 0}, d01 = {0, 0, 0, 0}, d11 = {0, 0, 0, 0};  // double: FP64 MFMA
-    if constexpr (sizeof(T) == 4) {
-        g.sums = sums;
-        g.zero(lane);
-    }
-#pragma unroll 1
-    for (int r0 = 0; r0 < rows; r0 += 64) {
-        double a[8][4];
+    for (int jt = 0; jt < NT; ++jt) {
+        double xs[8];
+        const int j = 16 * jt + li;
 #pragma unroll
-        for (int kg = 0; kg < 8; ++kg)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) a[kg][t] = (double)raw[kg].v[t];
-        const int r1 = r0 + 64;
-        if (r1 < rows) {
-#pragma unroll
-            for (int kg = 0; kg < 8; ++kg) {
-                const int k = 4 * kg + lq;
-                if (kg < nk) raw[kg] = load_raw4<T, VEC>(base, lda, r1 + 4 * li, k < n ? k : n - 1, rows, n);
-            }
+        for (int s = 0; s < 8; ++s) {
+            const int k = 8 * lq + s;
+            xs[s] = (k < n && j < n) ? X[k * 32 + j] : 0.0;
         }
-        v4d acc[4][2];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = v4d{0, 0, 0, 0};
-        // X is upper triangular: rows 16..31 are zero in columns 0..15
-#pragma unroll
-        for (int kg = 0; kg < 8; ++kg) {
-            if (kg < nk) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    if (kg < 4)
-                        acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg][t], mb[kg][0], acc[t][0], 0, 0, 0);
-                    if constexpr (NT == 2)
-                        acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg][t], mb[kg][1], acc[t][1], 0, 0, 0);
-                }
-            }
-        }
-        // q[jt][v][t] = Q[r0 + 16v + 4 lq + t][16 jt + li], rounded; zero in
-        // padded columns (rows past the end are zero already)
-        T q[2][4][4];
-#pragma unroll
-        for (int jt = 0; jt < 2; ++jt) {
-            const int cc = 16 * jt + li;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t) q[jt][v][t] = cc < n ? (T)acc[t][jt][v] : T(0);
-                if (cc < n)
-                    store_column<T, 4, VEC>(q[jt][v], base + (int64_t)cc * lda + r0 + 16 * v + 4 * lq, rows,
-                                            r0 + 16 * v + 4 * lq);
-            }
-        }
-        if constexpr (sizeof(T) == 4) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                float x0[8], x1[8];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    x0[t] = q[0][2 * h][t];
-                    x0[4 + t] = q[0][2 * h + 1][t];
-                    x1[t] = q[1][2 * h][t];
-                    x1[4 + t] = q[1][2 * h + 1][t];
-                }
-                g.add<NT>(x0, x1);
-            }
-            g.flush<NT>(lane);
-        } else {
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    d00 = __builtin_amdgcn_mfma_f64_16x16x4f64(q[0][v][t], q[0][v][t], d00, 0, 0, 0);
-                    if constexpr (NT == 2) {
-                        d01 = __builtin_amdgcn_mfma_f64_16x16x4f64(q[0][v][t], q[1][v][t], d01, 0, 0, 0);
-                        d11 = __builtin_amdgcn_mfma_f64_16x16x4f64(q[1][v][t], q[1][v][t], d11, 0, 0, 0);
-                    }
-                }
-        }
-    }
-    if constexpr (sizeof(T) == 4) {
-        g.store(Gout, lane);
-    } else {
-        // f64 accumulator: lane l, register v holds C[4v + l/16][l%16]
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = 4 * v + lq;
-            Gout[i * 32 + li] = d00[v];
-            Gout[i * 32 + 16 + li] = d01[v];
-            Gout[(16 + li) * 32 + i] = d01[v];
-            Gout[(16 + i) * 32 + 16 + li] = d11[v];
-        }
+        xb[jt] = split3(xs);
     }
 }
 
+// Gram accumulators of a Q pass: GramAcc32 for float, FP64 MFMA blocks
+// (0,0), (0,1), (1,1) for double.
+struct QGram {
+    GramAcc32 g;
+    v4d d00, d01, d11;
+};
+
+// The chunk in buf into the wave's LDS tile (column-major, pitch kTrPitch).
+__device__ __forceinline__ void q_tile_put(float* tw, const float (&buf)[8][4], int lane) {
+    const int li = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) tw[(4 * p + lq) * kTrPitch + 4 * li + t] = buf[p][t];
+}
+
+// FP32 Q chunk (q[jt][u][t], layout above) to A through the LDS tile, then
+// its BF16x3 Gram into g.
+template <int NT, bool VEC>
+__device__ __forceinline__ void q_chunk_finish32(float* tw, const float (&q)[2][4][4], float* __restrict__ base,
+                                                 int lda, int rows, int n, int r0, GramAcc32& g, int lane) {
+    const int li = lane & 15, lq = lane >> 4;
+    wave_sync();  // every lane's reads of the tile are done
+#pragma unroll
+    for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) tw[(16 * jt + li) * kTrPitch + 16 * u + 4 * lq + t] = q[jt][u][t];
+    wave_sync();
+#pragma unroll
+    for (int p = 0; p < 4 * NT; ++p) {
+        const int c = 4 * p + lq;
+        if (c < n) {
+            float v[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[t] = tw[c * kTrPitch + 4 * li + t];
+            store_column<float, 4, VEC>(v, base + (int64_t)c * lda + r0 + 4 * li, rows, r0 + 4 * li);
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        float x0[8], x1[8];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            x0[t] = q[0][2 * h][t];
+            x0[4 + t] = q[0][2 * h + 1][t];
+            x1[t] = q[NT - 1][2 * h][t];
+            x1[4 + t] = q[NT - 1][2 * h + 1][t];
+        }
+        g.add<NT>(x0, x1);
+    }
+    g.flush<NT>(lane);
+}
+
+// FP64 form of one chunk: a[kg][t] = buf[kg][t] converted (the caller may
+// reload buf meanwhile); Q to base (rows rows), Gram into acc.  tw: the
+// wave's LDS tile (float only).
 template <typename T, int NT, bool VEC>
-__global__ __launch_bounds__(64) void tsqr_mixed_q(const TreeDesc d, T* __restrict__ A, int lda,
-                                                   double* __restrict__ side, const int* __restrict__ ctl, int pass) {
-    __shared__ double sums[12 * 64];
+__device__ __forceinline__ void q_chunk_f64(const double (&a)[8][q_rpl<T>()], const double (&mb)[8][2], int nk,
+                                            T* __restrict__ base, int lda, int rows, int n, int r0, QGram& acc,
+                                            float* tw, int lane) {
+    constexpr int RPL = q_rpl<T>();
+    const int li = lane & 15, lq = lane >> 4;
+    v4d c[RPL][2];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) c[t][0] = c[t][1] = v4d{0, 0, 0, 0};
+    // X is upper triangular: rows 16..31 are zero in columns 0..15
+#pragma unroll
+    for (int kg = 0; kg < 8; ++kg) {
+        if (kg < nk) {
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                if (kg < 4) c[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg][t], mb[kg][0], c[t][0], 0, 0, 0);
+                if constexpr (NT == 2)
+                    c[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg][t], mb[kg][1], c[t][1], 0, 0, 0);
+            }
+        }
+    }
+    if constexpr (sizeof(T) == 4) {
+        float q[2][4][4];
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) {
+            const bool cin = 16 * jt + li < n;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) q[jt][v][t] = cin ? (float)c[t][jt][v] : 0.f;
+        }
+        q_chunk_finish32<NT, VEC>(tw, q, base, lda, rows, n, r0, acc.g, lane);
+    } else {
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) {
+            const int cc = 16 * jt + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                T q[RPL];
+#pragma unroll
+                for (int t = 0; t < RPL; ++t) q[t] = cc < n ? (T)c[t][jt][v] : T(0);
+                const int rr = r0 + RPL * (4 * v + lq);
+                if (cc < n) store_column<T, RPL, VEC>(q, base + (int64_t)cc * lda + rr, rows, rr);
+#pragma unroll
+                for (int t = 0; t < RPL; ++t) c[t][jt][v] = q[t];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                const double q0 = c[t][0][v];
+                acc.d00 = __builtin_amdgcn_mfma_f64_16x16x4f64(q0, q0, acc.d00, 0, 0, 0);
+                if constexpr (NT == 2) {
+                    const double q1 = c[t][1][v];
+                    acc.d01 = __builtin_amdgcn_mfma_f64_16x16x4f64(q0, q1, acc.d01, 0, 0, 0);
+                    acc.d11 = __builtin_amdgcn_mfma_f64_16x16x4f64(q1, q1, acc.d11, 0, 0, 0);
+                }
+            }
+    }
+}
+
+// BF16x3 form of one FP32 chunk, from the LDS tile: Q to base, Gram into g.
+template <int NT, bool VEC>
+__device__ __forceinline__ void q_chunk_bf16(float* tw, const Split3 (&xb)[NT], float* __restrict__ base, int lda,
+                                             int rows, int n, int r0, GramAcc32& g, int lane) {
+    const int li = lane & 15, lq = lane >> 4;
+    f4v c[4][NT];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+        float a[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) a[s] = tw[(8 * lq + s) * kTrPitch + 16 * rt + li];
+        const Split3 sa = split3(a);
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) c[rt][jt] = mma_bf16x3(sa, xb[jt], f4v{0, 0, 0, 0});
+    }
+    float q[2][4][4];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            q[0][rt][v] = c[rt][0][v];
+            q[1][rt][v] = c[rt][NT - 1][v];  // unused when NT == 1
+        }
+    q_chunk_finish32<NT, VEC>(tw, q, base, lda, rows, n, r0, g, lane);
+}
+
+// Q pass over superleaf `blk` of S of an m-row FP32 panel by one wave (the
+// batched path; the chunks j, j + S, ... as in mixed_gram_body), the next
+// chunk's loads in flight while the current one is multiplied; its Gram
+// matrix to Gout (FP64, row-major 32 x 32).  `bf16` picks the form; tw is
+// the wave's LDS tile.
+template <int NT, bool VEC>
+__device__ __forceinline__ void mixed_q_leaf(int m, int n, float* __restrict__ A, int lda,
+                                             const double* __restrict__ X, double* __restrict__ Gout, double* sums,
+                                             float* tw, int blk, int S, bool bf16) {
+    const int lane = threadIdx.x & 63;
+    const int chunks = (m + 63) / 64;
+    QGram acc;
+    acc.g.sums = sums;
+    acc.g.zero(lane);
+    float buf[8][4];
+    if (blk < chunks) q_chunk_load<float, VEC>(buf, A + (int64_t)blk * 64, lda, 0, m - 64 * blk, n, lane);
+    if (bf16) {
+        Split3 xb[NT];
+        q_x_bf16<NT>(xb, X, n, lane);
+#pragma unroll 1
+        for (int ch = blk; ch < chunks; ch += S) {
+            wave_sync();  // the previous chunk's LDS reads are done
+            q_tile_put(tw, buf, lane);
+            wave_sync();
+            const int nx = ch + S;
+            if (nx < chunks) q_chunk_load<float, VEC>(buf, A + (int64_t)nx * 64, lda, 0, m - 64 * nx, n, lane);
+            q_chunk_bf16<NT, VEC>(tw, xb, A + (int64_t)ch * 64, lda, m - 64 * ch, n, 0, acc.g, lane);
+        }
+    } else {
+        double mb[8][2];
+        q_x_f64(mb, X, lane);
+        const int nk = (n + 3) / 4;
+#pragma unroll 1
+        for (int ch = blk; ch < chunks; ch += S) {
+            double a[8][4];
+#pragma unroll
+            for (int kg = 0; kg < 8; ++kg)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) a[kg][t] = (double)buf[kg][t];
+            const int nx = ch + S;
+            if (nx < chunks) q_chunk_load<float, VEC>(buf, A + (int64_t)nx * 64, lda, 0, m - 64 * nx, n, lane);
+            q_chunk_f64<float, NT, VEC>(a, mb, nk, A + (int64_t)ch * 64, lda, m - 64 * ch, n, 0, acc, tw, lane);
+        }
+    }
+    acc.g.store(Gout, lane);
+}
+
+// Q pass of a single matrix, streamed by a persistent grid (as
+// tsqr_mixed_stream): workgroup b (4 waves) of P, wave w = 4b + 0..3 takes
+// the chunks w, w + 4P, ...  Each wave consumes its chunk's loads (into its
+// LDS tile or FP64 registers) before issuing the next chunk's.  The Gram
+// matrix of the stored Q is summed per wave (GramAcc32 slabs in LDS for
+// float, FP64 MFMA registers for double), then per workgroup in wave order
+// into partial[b], then per group (stream_finish); every sum has a fixed
+// order.  X: R^{-1} of pass 1, R2^{-1} of pass 2 (side block T0).
+template <typename T, int NT, bool VEC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void tsqr_mixed_qs(
+    int m, int n, T* __restrict__ A, int lda, const double* __restrict__ X, double* __restrict__ partial,
+    int* __restrict__ tk, const int* __restrict__ ctl, int pass) {
+    constexpr bool F32 = sizeof(T) == 4;
+    constexpr int RPL = q_rpl<T>();
+    constexpr int CR = 16 * RPL;  // rows per chunk
+    __shared__ double sums[F32 ? 4 : 1][12 * 64];
+    __shared__ float tr[F32 ? 4 : 1][32 * kTrPitch];
+    __shared__ GramLds L;
+    __shared__ int bcast;
     if (!ctl[0] || (pass == 2 && !ctl[1])) return;
-    // one workgroup per side superleaf (side_rows rows)
-    LevelDesc view = d.lv[0];
-    view.tile_rows = d.side_rows;
-    view.tiles = d.side_leaves;
-    mixed_q_body<T, NT, VEC>(view, d.n, A, lda, side + (int64_t)d.side_leaves * 1024,
-                             side + (int64_t)blockIdx.x * 1024, sums, blockIdx.x);
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wv = t >> 6;
+    const int W = 4 * (int)gridDim.x;
+    const int w = 4 * blockIdx.x + wv;
+    const int chunks = (m + CR - 1) / CR;
+    float* tw = tr[F32 ? wv : 0];
+    QGram acc;
+    acc.d00 = acc.d01 = acc.d11 = v4d{0, 0, 0, 0};
+    if constexpr (F32) {
+        acc.g.sums = sums[wv];
+        acc.g.zero(lane);
+    }
+    T buf[8][RPL];
+    if (w < chunks) q_chunk_load<T, VEC>(buf, A, lda, CR * w, m, n, lane);
+    bool bf16 = false;
+    if constexpr (F32) bf16 = ctl[pass == 1 ? kCtlBf16Q1 : kCtlBf16Q2] != 0;
+    if (bf16) {
+        if constexpr (F32) {
+            Split3 xb[NT];
+            q_x_bf16<NT>(xb, X, n, lane);
+#pragma unroll 1
+            for (int ch = w; ch < chunks; ch += W) {
+                wave_sync();  // the previous chunk's LDS reads are done
+                q_tile_put(tw, buf, lane);
+                wave_sync();
+                if (ch + W < chunks) q_chunk_load<T, VEC>(buf, A, lda, CR * (ch + W), m, n, lane);
+                q_chunk_bf16<NT, VEC>(tw, xb, A, lda, m, n, CR * ch, acc.g, lane);
+            }
+        }
+    } else {
+        double mb[8][2];
+        q_x_f64(mb, X, lane);
+        const int nk = (n + 3) / 4;
+#pragma unroll 1
+        for (int ch = w; ch < chunks; ch += W) {
+            double a[8][RPL];
+#pragma unroll
+            for (int kg = 0; kg < 8; ++kg)
+#pragma unroll
+                for (int t4 = 0; t4 < RPL; ++t4) a[kg][t4] = (double)buf[kg][t4];
+            if (ch + W < chunks) q_chunk_load<T, VEC>(buf, A, lda, CR * (ch + W), m, n, lane);
+            q_chunk_f64<T, NT, VEC>(a, mb, nk, A, lda, m, n, CR * ch, acc, tw, lane);
+        }
+    }
+    if constexpr (F32)
+        gram32_finish<NT>(L, sums, partial, tk, &bcast);
+    else
+        stream_reduce<NT>(L, acc.d00, acc.d01, acc.d11, partial, tk, &bcast);
 }
 
 // Residual check after pass `pass`, by one wave (lanes 0..63), on the Gram
@@ -480,7 +719,10 @@ __device__ __forceinline__ void mixed_check_body(GramLds& L, int n, T* __restric
     if (lane < 32) {
 #pragma unroll
         for (int r = 0; r < 32; ++r) X[r * 32 + c] = x[r];
-        if (lane == 0) ctl[1] = 1;
+        if (lane == 0) {
+            ctl[1] = 1;
+            ctl[kCtlBf16Q2] = (e <= kBf16QMaxResidual) ? 1 : 0;
+        }
     }
 }
 

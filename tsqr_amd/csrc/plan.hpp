@@ -39,12 +39,10 @@ struct TreeDesc {
     int arity;   // children per parent
     int levels;  // number of levels (root is levels - 1)
     int total_ctr;
-    // FP64 32 x 32 side blocks kept per superleaf of side_rows rows (the
-    // superleaf Gram paths and the mixed Q pass), then the whole-matrix
-    // blocks.  Superleaf plans: the level-0 tiles themselves.  Plans with
-    // single-tile leaves: none (AUTO, HOUSEHOLDER), or superleaves of
-    // side_rows rows that group several leaves (CHOLQR2's Q pass).
-    int side_leaves, side_rows;
+    // FP64 32 x 32 side blocks kept per level-0 superleaf (the superleaf
+    // Gram paths), then the whole-matrix blocks; none per leaf in plans with
+    // single-tile leaves.
+    int side_leaves;
     int64_t block_elems;  // elements of all blocks
     LevelDesc lv[kMaxLevels];
 };
@@ -93,7 +91,6 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d, int arity
     d->levels = l + 1;
     d->total_ctr = ctr;
     d->side_leaves = d->lv[0].tiles;
-    d->side_rows = leaf_rows;
     d->block_elems = blk;
     return true;
 }
@@ -104,9 +101,9 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d, int arity
 //   flags    one int per level-0 superleaf: 0 Gram path, 1 Householder path
 //   list     count, then the indices of the Householder-path superleaves
 //   ctl      control words: ctl[0] = 1 when the whole matrix took the flat
-//            Gram path; ctl[1], ctl[2] for the mixed path (mixed.hpp);
-//            ctl[64 .. 128) and ctl[128 .. 192) the tickets of two streaming
-//            Gram passes
+//            Gram path; ctl[1] .. ctl[4] for the mixed path (mixed.hpp);
+//            ctl[64 .. 128) .. ctl[256 .. 320) the tickets of four streaming
+//            passes (the Gram passes and the mixed path's Q passes)
 //   partial  32 x 32 FP64 partial sums: min(superleaves, kGramPartials)
 //            blocks of superleaf Gram sums plus kGramPartials2 for a second
 //            stage when there are more than kGramPartials2 of them, or the
@@ -118,9 +115,18 @@ inline bool make_tree(int m, int n, int H, int leaf_rows, TreeDesc* d, int arity
 //            R (T0+1) and the mixed path's residuals (T0+2)
 constexpr int kGramPartials = 1024;
 constexpr int kGramPartials2 = 32;
-constexpr int kCtlWords = 192;          // control words, including the stream tickets
+constexpr int kCtlWords = 320;          // control words, including the stream tickets
 constexpr int kCtlStreamTickets = 64;   // ctl[64 ..]: tickets of the streaming Gram
 constexpr int kCtlStreamTickets2 = 128; // ctl[128 ..]: tickets of a second streaming pass
+constexpr int kCtlStreamTickets3 = 192; // ctl[192 ..], ctl[256 ..]: tickets of the mixed path's
+constexpr int kCtlStreamTickets4 = 256; //   two streamed Q passes
+// ctl[3], ctl[4]: the FP32 mixed path's Q pass 1 / pass 2 may run on the BF16
+// matrix cores (set where R^{-1} is well conditioned: kappa_est <= 2n after
+// the first Cholesky, ||Q1^T Q1 - I||_F <= kBf16QMaxResidual before the
+// second)
+constexpr int kCtlBf16Q1 = 3;
+constexpr int kCtlBf16Q2 = 4;
+constexpr double kBf16QMaxResidual = 0.5;
 
 // The streaming Gram kernel (tsqr_gram_stream, gram.hpp) runs P workgroups
 // of 4 waves, P = stream_groups(m): at most kStreamMaxGroups, and four

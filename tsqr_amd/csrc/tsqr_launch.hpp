@@ -59,11 +59,9 @@ inline int leaf_rows_for(int m, int n) {
 // Plans the tree of a call on leaf path `path` (LeafPath):
 //   GRAM_TREE    superleaves of leaf_rows_for rows, each with a side block
 //                (its Gram matrix, then R^{-1});
-//   AUTO, HOUSEHOLDER  leaves of one tile (the Householder kernels of
-//                leaf1.hpp), no per-leaf side blocks;
-//   CHOLQR2      leaves of one tile, and side blocks for superleaves of
-//                leaf_rows_for rows (the mixed Q pass sums its Gram per
-//                superleaf).
+//   AUTO, HOUSEHOLDER, CHOLQR2  leaves of one tile (the Householder
+//                kernels of leaf1.hpp), no per-leaf side blocks (the flat
+//                passes stream the whole matrix).
 template <typename T>
 inline bool plan_tsqr(int m, int n, TreeDesc* d, int path) {
     const int N = padded_cols(n);
@@ -71,12 +69,7 @@ inline bool plan_tsqr(int m, int n, TreeDesc* d, int path) {
     const int sl = leaf_rows_for<T>(m, n);
     const bool tiles = path != 2;  // kLeafGramTree (LeafPath, below)
     if (!make_tree(m, n, H, tiles ? H : sl, d, tuned_tree_arity(sizeof(T) == 4 ? 0 : 1, N))) return false;
-    if (path == 0 || path == 1) {  // kLeafAuto, kLeafHouseholder
-        d->side_leaves = 0;
-    } else if (path == 3) {  // kLeafCholQR2
-        d->side_rows = sl;
-        d->side_leaves = (int)(((int64_t)m + sl - 1) / sl);
-    }
+    if (path != 2) d->side_leaves = 0;  // all but kLeafGramTree
     return true;
 }
 

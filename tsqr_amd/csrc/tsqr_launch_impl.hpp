@@ -19,11 +19,12 @@
 //   gram_q        M = R^{-1} S, then Q = A M, for the Gram superleaves
 //                 (FP32 flat path: flat_q4, Q = A R^{-1} by a persistent grid)
 //   super_q       Householder Q for the flagged superleaves
-// The CholeskyQR2 path replaces the Gram kernels by mixed_gram (FP32,
-// BF16x3) with the flat reduction and Cholesky, or gram_stream (FP64), then mixed_q +
-// mixed_check (pass 1) and mixed_q + mixed_check (pass 2, a no-op unless
-// the check asks for it); every superleaf is flagged Householder so that the
-// tree path takes over if the flat gate rejects.
+// The CholeskyQR2 path replaces the Gram kernels by mixed_stream (FP32,
+// BF16x3) or gram_stream (FP64) with the flat reduction and Cholesky, then
+// the streamed Q pass mixed_qs + mixed_check (pass 1) and mixed_qs +
+// mixed_check (pass 2, a no-op unless the check asks for it); every leaf is
+// flagged Householder so that the tree path takes over if the flat gate
+// rejects.
 // A single tile (m <= H) is one tsqr_single launch.
 // R only (LaunchOpts::r_only, the tsqr_r_* entry points): the sequence stops
 // after the tree walk up, so A is only read.  The FP64 CholeskyQR2 path then
@@ -66,28 +67,6 @@ __global__ void init_flags(int* __restrict__ tickets, int nt, int* __restrict__ 
     if (i == 0) list[0] = value ? nf : 0;
 }
 
-// Sums `count` 32 x 32 FP64 blocks at src into at most kGramPartials2
-// partials (two deterministic stages); returns the partials and their count.
-inline const double* reduce_grams(int count, const double* src, double* partial, const int* run_if, int* nparts,
-                                  hipStream_t st) {
-    if (count <= kGramPartials2 * kGramPartials2) {  // one stage: at most 32 blocks per workgroup
-        const int p = count < kGramPartials2 ? count : kGramPartials2;
-        hipLaunchKernelGGL(tsqr_gram_partial<>, dim3(p), dim3(256), 0, st, count, src, partial, run_if);
-        *nparts = p;
-        return partial;
-    }
-    const int p1 = count < kGramPartials ? count : kGramPartials;
-    hipLaunchKernelGGL(tsqr_gram_partial<>, dim3(p1), dim3(256), 0, st, count, src, partial, run_if);
-    if (p1 <= kGramPartials2) {
-        *nparts = p1;
-        return partial;
-    }
-    double* second = partial + (size_t)p1 * 1024;
-    hipLaunchKernelGGL(tsqr_gram_partial<>, dim3(kGramPartials2), dim3(256), 0, st, p1, (const double*)partial,
-                       second, run_if);
-    *nparts = kGramPartials2;
-    return second;
-}
 }  // namespace
 
 template <typename T, int N, bool VEC>
@@ -144,7 +123,7 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
             hipLaunchKernelGGL((tsqr_mixed_stream<NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const float*)A, lda,
                                partial, stream_tk);
             hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
-                               (const double*)stream_sum, R, ldr, side, ctl, o.gate, rkeep);
+                               (const double*)stream_sum, R, ldr, side, ctl, o.gate, rkeep, 1);
         } else {
             hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
                                partial, stream_tk);
@@ -259,13 +238,15 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     if (timed) timing_mark(kPhaseDown, false, st);
     if (timed) timing_mark(kPhaseLeafQ, true, st);
     if (mixed) {
+        // streamed Q passes (the grid of the 4 x 4 R pass: its partial sums
+        // fit the workspace)
+        const int qgroups = stream_groups4(d.m);
         for (int pass = 1; pass <= 2; ++pass) {
-            hipLaunchKernelGGL((tsqr_mixed_q<T, NT, VEC>), dim3(d.side_leaves), dim3(64), 0, st, d, A, lda, side,
-                               (const int*)ctl, pass);
-            int np = 0;
-            const double* parts = reduce_grams(d.side_leaves, side, partial, ctl + (pass == 1 ? 0 : 1), &np, st);
-            hipLaunchKernelGGL(tsqr_mixed_check<T>, dim3(1), dim3(kSumThreads), 0, st, d, np, parts, R, ldr, side, ctl, pass,
-                               o.tol);
+            hipLaunchKernelGGL((tsqr_mixed_qs<T, NT, VEC>), dim3(qgroups), dim3(256), 0, st, d.m, d.n, A, lda, X0,
+                               partial, ctl + (pass == 1 ? kCtlStreamTickets3 : kCtlStreamTickets4), (const int*)ctl,
+                               pass);
+            hipLaunchKernelGGL(tsqr_mixed_check<T>, dim3(1), dim3(kSumThreads), 0, st, d, stream_group_count(qgroups),
+                               (const double*)(partial + (size_t)qgroups * 1024), R, ldr, side, ctl, pass, o.tol);
         }
     }
     if (flat_auto && sizeof(T) == 4) {
