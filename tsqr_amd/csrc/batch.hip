@@ -89,6 +89,7 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
             c.leaf_rows = super_tiles<T>(32) * 64 * rows_per_lane<T>(32);
             c.leaves = (m[i] + c.leaf_rows - 1) / c.leaf_rows;
             c.vec = p.vec;
+            c.mixed = (sizeof(T) == 4 && b->method == TSQR_BATCH_MIXED) ? 1 : 0;
             c.ws = (int64_t)off;
             off += couple_ws_bytes(c.leaves);
             couples.push_back(c);
@@ -122,7 +123,7 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
 
     // item lists (problem indices into `all`)
     struct Stage {
-        std::vector<int2> probs, leaves, level1;
+        std::vector<int2> probs, leaves, groups, level1;
         std::vector<std::vector<int2>> down, up;
     } st[2];
     for (int k = 0; k < (int)all.size(); ++k) {
@@ -130,6 +131,7 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
         const TreeDesc& d = all[k].d;
         s.probs.push_back(make_int2(k, 0));
         for (int t = 0; t < d.lv[0].tiles; ++t) s.leaves.push_back(make_int2(k, t));
+        for (int g = 0; g < batch_groups(d.lv[0].tiles); ++g) s.groups.push_back(make_int2(k, g));
         if (d.levels >= 2)
             for (int t = 0; t < d.lv[1].tiles; ++t) s.level1.push_back(make_int2(k, t));
         for (int l = 1; l <= d.levels - 2; ++l) {
@@ -144,14 +146,14 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
     std::vector<int2> crow, clist;
     for (int c = 0; c < (int)couples.size(); ++c) {
         clist.push_back(make_int2(c, 0));
-        for (int t = 0; t < couples[c].leaves; ++t) crow.push_back(make_int2(c, t));
+        for (int g = 0; g < batch_groups(couples[c].leaves); ++g) crow.push_back(make_int2(c, g));
     }
 
     // one device allocation: problems, couples, then every item list
     size_t bytes = align256(all.size() * sizeof(BatchProb)) + align256(couples.size() * sizeof(BatchCouple));
     size_t nitems = crow.size() + clist.size();
     for (auto& s : st) {
-        nitems += s.probs.size() + s.leaves.size() + s.level1.size();
+        nitems += s.probs.size() + s.leaves.size() + s.groups.size() + s.level1.size();
         for (auto& d : s.down) nitems += d.size();
         for (auto& u : s.up) nitems += u.size();
     }
@@ -175,11 +177,12 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
     auto put_items = [&](const std::vector<int2>& v) { return At{put(v.data(), v.size() * sizeof(int2)), (int)v.size()}; };
     At a_crow = put_items(crow);
     At a_clist = put_items(clist);
-    At a_probs[2], a_leaves[2], a_l1[2];
+    At a_probs[2], a_leaves[2], a_groups[2], a_l1[2];
     std::vector<At> a_down[2], a_up[2];
     for (int s = 0; s < 2; ++s) {
         a_probs[s] = put_items(st[s].probs);
         a_leaves[s] = put_items(st[s].leaves);
+        a_groups[s] = put_items(st[s].groups);
         a_l1[s] = put_items(st[s].level1);
         for (auto& d : st[s].down) a_down[s].push_back(put_items(d));
         for (auto& u : st[s].up) a_up[s].push_back(put_items(u));
@@ -204,6 +207,7 @@ int plan(tsqr_batch* b, const int* m, const int* n, void* const* A, const int* l
         dev.stage[s].fallback = s == 0 ? 0 : (int64_t)list_bytes[0];
         dev.stage[s].probs = list(a_probs[s]);
         dev.stage[s].leaves = list(a_leaves[s]);
+        dev.stage[s].groups = list(a_groups[s]);
         dev.stage[s].level1 = list(a_l1[s]);
         dev.stage[s].down.clear();
         for (auto& a : a_down[s]) dev.stage[s].down.push_back(list(a));

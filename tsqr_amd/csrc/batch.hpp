@@ -73,6 +73,7 @@ struct BatchCouple {
     int m, n2, lda, ldr;
     int leaves, leaf_rows;
     int vec;
+    int mixed;      // 1: mixed method (FP32): BF16x3 coupling products
     int gram;       // 1: batch_project also writes the BF16x3 Gram matrix of
                     // each new A2 row block, the second panel's superleaf b,
                     // to the second panel's side block b (at byte side2)
@@ -82,14 +83,24 @@ struct BatchCouple {
 // Per-couple workspace: block b < leaves is row block b's product, block
 // `leaves` the summed product P (the projection's right factor), block
 // `leaves + 1` the accumulated R12 in FP64; then the squared column norms of
-// A2 per row block (32 doubles each); then 16 ints: tickets of the two
-// passes and the reorthogonalisation flag.
+// A2 per row block (32 doubles each); then kCoupleInts ints (the
+// reorthogonalisation flag).
 __host__ __device__ inline size_t couple_norms_off(int leaves) { return ((size_t)leaves + 2) * 1024 * sizeof(double); }
 __host__ __device__ inline size_t couple_ints_off(int leaves) {
     return couple_norms_off(leaves) + (size_t)leaves * 32 * sizeof(double);
 }
-inline size_t couple_ws_bytes(int leaves) { return align256(couple_ints_off(leaves) + 64); }
 constexpr int kCoupleReorth = 4;  // int index of the flag
+constexpr int kCoupleInts = 16;
+inline size_t couple_ws_bytes(int leaves) { return align256(couple_ints_off(leaves) + kCoupleInts * sizeof(int)); }
+
+// Row blocks of the streaming batch passes (Gram, Q, coupling products and
+// projection): a panel with S superleaves (a couple with S row blocks) is
+// worked on by batch_groups(S) workgroups of four waves; wave w of
+// workgroup g takes the rows of block 4g + w of 4 batch_groups(S) (the
+// strided chunks of mixed_gram_body), and the workgroup adds its four
+// waves' Gram or product blocks in wave order into block g, so the
+// per-matrix sums read batch_groups(S) blocks.
+__host__ __device__ inline int batch_groups(int S) { return (S + 1) / 2; }
 
 template <int V>
 using IntC = std::integral_constant<int, V>;
@@ -129,13 +140,14 @@ constexpr int batch_super_threads() {
     return 64 * (a > b ? (a > c ? a : c) : (b > c ? b : c));
 }
 
-// L.G = the sum of `count` 32 x 32 FP64 blocks at src, by a workgroup of
-// four waves (256 threads): wave w adds the blocks [w c / 4, (w + 1) c / 4)
-// in order, four blocks in flight (they were written by other CUs, so every
-// load goes to memory), then the four wave sums are added in wave order in
-// LDS, so the result is fixed for a given count.  Every thread of the
-// workgroup must call it.
-__device__ __forceinline__ void sum_blocks_wg4(GramLds& L, const double* __restrict__ src, int count) {
+// L.G = the sum of `count` 32 x 32 FP64 blocks at src (block b at b * stride
+// blocks), by a workgroup of four waves (256 threads): wave w adds the
+// blocks [w c / 4, (w + 1) c / 4) in order, four blocks in flight (they were
+// written by other CUs, so every load goes to memory), then the four wave
+// sums are added in wave order in LDS, so the result is fixed for a given
+// count.  Every thread of the workgroup must call it.
+__device__ __forceinline__ void sum_blocks_wg4(GramLds& L, const double* __restrict__ src, int count,
+                                               int stride = 1) {
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int b0 = (int)((int64_t)count * w / 4), b1 = (int)((int64_t)count * (w + 1) / 4);
@@ -149,7 +161,7 @@ __device__ __forceinline__ void sum_blocks_wg4(GramLds& L, const double* __restr
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
-            for (int e = 0; e < 16; ++e) v[q][e] = src[(int64_t)(b + q) * 1024 + lane + 64 * e];
+            for (int e = 0; e < 16; ++e) v[q][e] = src[(int64_t)(b + q) * stride * 1024 + lane + 64 * e];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -158,7 +170,7 @@ __device__ __forceinline__ void sum_blocks_wg4(GramLds& L, const double* __restr
 #pragma unroll 1
     for (; b < b1; ++b)
 #pragma unroll
-        for (int e = 0; e < 16; ++e) s[e] += src[(int64_t)b * 1024 + lane + 64 * e];
+        for (int e = 0; e < 16; ++e) s[e] += src[(int64_t)b * stride * 1024 + lane + 64 * e];
 #pragma unroll 1
     for (int k = 0; k < 4; ++k) {
         if (w == k) {
@@ -173,7 +185,7 @@ __device__ __forceinline__ void sum_blocks_wg4(GramLds& L, const double* __restr
 }
 
 // ctl words of a problem (int): 0 flat CholeskyQR taken, 1 second pass
-// asked, 2 check status.
+// asked, 2 check status, 3 / 4 BF16 Q passes (mixed.hpp).
 
 // ---------------------------------------------------------------------------
 // Problem set-up: tickets and control words = 0; couples' tickets = 0;
@@ -196,14 +208,14 @@ __global__ __launch_bounds__(256) void batch_init(const BatchProb* __restrict__ 
         int* tickets = reinterpret_cast<int*>(ws + p.w.tickets);
         int* ctl = reinterpret_cast<int*>(ws + p.w.ctl);
         for (int i = threadIdx.x; i < p.d.total_ctr; i += blockDim.x) tickets[i] = 0;
-        if (threadIdx.x < 64) ctl[threadIdx.x] = 0;
+        for (int i = threadIdx.x; i < kCtlWords; i += blockDim.x) ctl[i] = 0;
         return;
     }
     const int c = b - nprobs;
     if (c >= ncouples) return;
     const BatchCouple& q = couples[c];
     int* tk = reinterpret_cast<int*>(work + q.ws + couple_ints_off(q.leaves));
-    if (threadIdx.x < 16) tk[threadIdx.x] = 0;
+    if (threadIdx.x < kCoupleInts) tk[threadIdx.x] = 0;
     // R21: rows [32, 32 + n2) of columns [0, 32), i.e. relative to R12 at
     // (0, 32): rows 32.., columns -32..-1
     T* R21 = reinterpret_cast<T*>(q.R12) + 32 - (int64_t)32 * q.ldr;
@@ -381,21 +393,27 @@ __device__ __forceinline__ double* prob_side(const BatchProb& p, char* work) {
     return reinterpret_cast<double*>(prob_ws(p, work) + p.w.side);
 }
 
+// Item (problem, g), g < batch_groups(S): four waves, wave w on row block
+// 4g + w of 4 batch_groups(S) (mixed_gram_body's strided chunks); the
+// workgroup adds its four Gram slabs in wave order into side block g.
 template <int UNUSED = 0>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void batch_mixed_gram(const BatchProb* __restrict__ probs,
-                                                        const int2* __restrict__ items, char* __restrict__ work) {
-    __shared__ double sums[12 * 64];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void batch_mixed_gram(
+    const BatchProb* __restrict__ probs, const int2* __restrict__ items, char* __restrict__ work) {
+    __shared__ double sums[4][12 * 64];
     const int2 it = items[blockIdx.x];
     const BatchProb& p = probs[it.x];
     if (p.pre_gram) return;  // written by batch_project
     double* side = prob_side(p, work);
-    const LevelDesc& L0 = p.d.lv[0];
+    const int G = batch_groups(p.d.lv[0].tiles);
+    const int wv = threadIdx.x >> 6;
     const int n = p.d.n;
     dispatch_nv(p.N, p.vec, [&](auto NC, auto VC) {
         constexpr int NT = decltype(NC)::value > 16 ? 2 : 1;
-        mixed_gram_body<NT, decltype(VC)::value>(p.d.m, n, reinterpret_cast<const float*>(p.A), p.lda,
-                                                 side + (int64_t)it.y * 1024, sums, it.y, L0.tiles);
+        mixed_gram_body<NT, decltype(VC)::value>(p.d.m, n, reinterpret_cast<const float*>(p.A), p.lda, sums[wv],
+                                                 4 * it.y + wv, 4 * G);
     });
+    __syncthreads();
+    gram32_combine(sums, side + (int64_t)it.y * 1024);
 }
 
 // One workgroup per problem, after batch_mixed_gram: the sum of its
@@ -415,7 +433,7 @@ __global__ __launch_bounds__(256) void batch_mixed_flat(const BatchProb* __restr
     double* side = prob_side(p, work);
     const LevelDesc& L0 = p.d.lv[0];
     int* ctl = reinterpret_cast<int*>(ws + p.w.ctl);
-    sum_blocks_wg4(L, side, L0.tiles);
+    sum_blocks_wg4(L, side, batch_groups(L0.tiles));
     if (threadIdx.x >= 64) return;
     const int64_t T0 = L0.tiles;
     const bool ok = flat_factor<float>(L, p.d.n, reinterpret_cast<float*>(p.R), p.ldr, side + T0 * 1024, ctl, gate,
@@ -429,28 +447,31 @@ __global__ __launch_bounds__(256) void batch_mixed_flat(const BatchProb* __restr
     }
 }
 
-// (at least two waves per SIMD: the register budget of the single-matrix
-// tsqr_mixed_q)
+// Item (problem, g) by four waves, row blocks as in batch_mixed_gram; the
+// Gram slabs of the stored rows go to side block g.
+// (Two waves per SIMD: the register budget of the single-matrix Q pass.)
 template <int UNUSED = 0>
-__global__ __launch_bounds__(64, 2) void batch_mixed_q(const BatchProb* __restrict__ probs,
-                                                     const int2* __restrict__ items, char* __restrict__ work,
-                                                     int pass) {
-    __shared__ double sums[12 * 64];
-    __shared__ float tr[32 * kTrPitch];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_mixed_q(
+    const BatchProb* __restrict__ probs, const int2* __restrict__ items, char* __restrict__ work, int pass) {
+    __shared__ double sums[4][12 * 64];
+    __shared__ float tr[4][32 * kTrPitch];
     const int2 it = items[blockIdx.x];
     const BatchProb& p = probs[it.x];
     char* ws = prob_ws(p, work);
     const int* ctl = reinterpret_cast<const int*>(ws + p.w.ctl);
     if (!p.mixed || !ctl[0] || (pass == 2 && !ctl[1])) return;
     double* side = prob_side(p, work);
-    const LevelDesc& L0 = p.d.lv[0];
-    const int64_t T0 = L0.tiles;
+    const int64_t T0 = p.d.lv[0].tiles;
+    const int G = batch_groups((int)T0);
+    const int wv = threadIdx.x >> 6;
     const bool bf16q = ctl[pass == 1 ? kCtlBf16Q1 : kCtlBf16Q2] != 0;
     dispatch_nv(p.N, p.vec, [&](auto NC, auto VC) {
         constexpr int NT = decltype(NC)::value > 16 ? 2 : 1;
         mixed_q_leaf<NT, decltype(VC)::value>(p.d.m, p.d.n, reinterpret_cast<float*>(p.A), p.lda, side + T0 * 1024,
-                                              side + (int64_t)it.y * 1024, sums, tr, it.y, (int)T0, bf16q);
+                                              sums[wv], tr[wv], 4 * it.y + wv, 4 * G, bf16q);
     });
+    __syncthreads();
+    gram32_combine(sums, side + (int64_t)it.y * 1024);
 }
 
 // Residual check of one problem after a Q pass (one workgroup per problem:
@@ -468,33 +489,32 @@ __global__ __launch_bounds__(256) void batch_mixed_check(const BatchProb* __rest
     if (!p.mixed || !ctl[0] || (pass == 2 && !ctl[1])) return;
     double* side = prob_side(p, work);
     const int64_t T0 = p.d.lv[0].tiles;
-    sum_blocks_wg4(L, side, (int)T0);
+    sum_blocks_wg4(L, side, batch_groups((int)T0));
     if (threadIdx.x >= 64) return;
     mixed_check_body<float>(L, p.d.n, reinterpret_cast<float*>(p.R), p.ldr, side + T0 * 1024,
                             side + (T0 + 1) * 1024, side + (T0 + 2) * 1024, ctl, pass, tol, lane);
 }
 
 // ---------------------------------------------------------------------------
-// Coupling of two panels: P = Q1^T A2 over a couple's rows (FP64 MFMA, FP32
-// or FP64 inputs exact in FP64), summed per couple into R12
-// (batch_cross_sum); then A2 -= Q1 P.  `pass` 2 (mixed method) adds the
-// second product to R12.
+// Coupling of two panels: P = Q1^T A2 over a couple's rows (plain method:
+// FP64 MFMA, FP32 or FP64 inputs exact in FP64; mixed method: BF16x3 with
+// FP32 sums per 64 rows), summed per couple into R12 (batch_cross_sum); then
+// A2 -= Q1 P.  `pass` 2 (mixed method) adds the second product to R12.
 // ---------------------------------------------------------------------------
 
-// P_b = Q1_b^T A2_b for row block `blk` of q.leaves, written to out (FP64
-// 32 x 32, row-major), and the block's squared column norms of A2 to norms.
-// The row blocks partition the 64-row chunks by stride (block j: chunks j,
-// j + S, ..., S = q.leaves), so a couple's waves sweep it together; only
-// the sums over all blocks are used.  Lane l loads rows 4(l/16)..+3 of
+// P_b = Q1_b^T A2_b for row block `blk` of S, written to out (FP64 32 x 32,
+// row-major), and the block's squared column norms of A2 to norms.  The row
+// blocks partition the 64-row chunks by stride (block j: chunks j, j + S,
+// ...), so a couple's waves sweep it together; only the sums over all
+// blocks are used.  Lane l loads rows 4(l/16)..+3 of
 // columns l%16 and 16 + l%16 of both matrices in each of a chunk's four
 // 16-row slabs; the next chunk is in flight while the current one is
 // multiplied.
 template <typename T, bool VEC>
-__device__ __forceinline__ void cross_block(const BatchCouple& q, int blk, double* __restrict__ out,
-                                            double* __restrict__ norms, int lane) {
+__device__ __forceinline__ void cross_block(const BatchCouple& q, int blk, int S, double* out, double* norms,
+                                            int lane) {
     const int col = lane & 15;
     const int rq = 4 * (lane >> 4);
-    const int S = q.leaves;
     const int chunks = (q.m + 63) / 64;
     const T* Q1 = reinterpret_cast<const T*>(q.Q1);
     const T* A2 = reinterpret_cast<const T*>(q.A2);
@@ -565,27 +585,139 @@ __device__ __forceinline__ void cross_block(const BatchCouple& q, int blk, doubl
     }
 }
 
+// The mixed method's coupling product (FP32 panels): as cross_block, with
+// BF16x3 products (bf16x3.hpp) summed in FP32 over each 64-row chunk and the
+// chunk sums added in FP64 (the accuracy of the mixed method's Gram
+// matrices).  Lane l loads rows 8(l/16) .. +7 of columns l%16 and 16 + l%16
+// of Q1 and A2 in each 32-row step, the same rows feeding the same BF16
+// k-slots of both operands; the next chunk is in flight meanwhile.  Output
+// and norms as cross_block's.
+template <bool VEC>
+__device__ __forceinline__ void cross_block_bf16(const BatchCouple& q, int blk, int S, double* out, double* norms,
+                                                 int lane) {
+    const int col = lane & 15;
+    const int rq = 8 * (lane >> 4);
+    const int chunks = (q.m + 63) / 64;
+    const float* Q1 = reinterpret_cast<const float*>(q.Q1);
+    const float* A2 = reinterpret_cast<const float*>(q.A2);
+    double p64[4][4];  // [block (Q1 half, A2 half)][register]
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) p64[b][v] = 0.0;
+    double s0sq = 0.0, s1sq = 0.0;
+    // x[h][0..3]: Q1 columns col, col + 16, A2 columns col, col + 16 of step h
+    auto load = [&](int ch, float(&x)[2][4][8]) {
+        const int64_t r0 = (int64_t)ch * 64;
+        const int lim = (int)(q.m - r0 < 64 ? q.m - r0 : 64);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            load8<VEC>(x[h][0], Q1 + r0, q.lda, 32 * h + rq, col, lim, 32);
+            load8<VEC>(x[h][1], Q1 + r0, q.lda, 32 * h + rq, col + 16, lim, 32);
+            load8<VEC>(x[h][2], A2 + r0, q.lda, 32 * h + rq, col, lim, q.n2);
+            load8<VEC>(x[h][3], A2 + r0, q.lda, 32 * h + rq, col + 16, lim, q.n2);
+        }
+    };
+    float cur[2][4][8];
+    int ch = blk;
+    if (ch < chunks) load(ch, cur);
+#pragma unroll 1
+    for (; ch < chunks; ch += S) {
+        const int nx = ch + S;
+        float nxt[2][4][8];
+        if (nx < chunks) load(nx, nxt);
+        f4v c[4] = {f4v{0, 0, 0, 0}, f4v{0, 0, 0, 0}, f4v{0, 0, 0, 0}, f4v{0, 0, 0, 0}};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                s0sq = fmat((double)cur[h][2][e], (double)cur[h][2][e], s0sq);
+                s1sq = fmat((double)cur[h][3][e], (double)cur[h][3][e], s1sq);
+            }
+            const Split3 q0 = split3(cur[h][0]), q1 = split3(cur[h][1]);
+            const Split3 a0 = split3(cur[h][2]), a1 = split3(cur[h][3]);
+            c[0] = mma_bf16x3(q0, a0, c[0]);
+            c[1] = mma_bf16x3(q0, a1, c[1]);
+            c[2] = mma_bf16x3(q1, a0, c[2]);
+            c[3] = mma_bf16x3(q1, a1, c[3]);
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) p64[b][v] += (double)c[b][v];
+        if (nx >= chunks) break;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) cur[h][a][e] = nxt[h][a][e];
+    }
+    // f32 accumulator: lane l, register v holds C[4(l/16) + v][l%16]
+    const int i0 = 4 * (lane >> 4);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int i = i0 + v;
+        out[i * 32 + col] = p64[0][v];
+        out[i * 32 + 16 + col] = p64[1][v];
+        out[(16 + i) * 32 + col] = p64[2][v];
+        out[(16 + i) * 32 + 16 + col] = p64[3][v];
+    }
+    s0sq = fold_all<0>(fold_all<1>(s0sq));
+    s1sq = fold_all<0>(fold_all<1>(s1sq));
+    if (lane < 16) {
+        norms[col] = s0sq;
+        norms[16 + col] = s1sq;
+    }
+}
+
 // Pass 1 also decides whether the second pass is needed ("twice is enough",
 // Kahan-Parlett): with ||a_j||^2 the squared norm of column j of A2 and
 // P_j the product's column j, ||a_j - Q1 P_j||^2 = ||a_j||^2 - ||P_j||^2
 // (Q1 orthonormal); if every column keeps at least half its squared norm,
 // one projection leaves A2 orthogonal to Q1 to the working precision and the
 // second pass is skipped.  (The plain method never runs a second pass.)
+// Item (couple, g), g < batch_groups(q.leaves), by four waves: wave w takes
+// row block 4g + w of 4 batch_groups(q.leaves); the workgroup adds the four
+// products and norm vectors in wave order into block g.
 template <typename T>
-__global__ __launch_bounds__(64) void batch_cross(const BatchCouple* __restrict__ couples,
-                                                   const int2* __restrict__ items, char* __restrict__ work,
-                                                   int pass) {
-    const int lane = threadIdx.x;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void batch_cross(
+    const BatchCouple* __restrict__ couples, const int2* __restrict__ items, char* __restrict__ work, int pass) {
+    __shared__ double red[4][1024 + 32];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wv = t >> 6;
     const int2 it = items[blockIdx.x];
     const BatchCouple& q = couples[it.x];
     double* blocks = reinterpret_cast<double*>(work + q.ws);
     double* norms = reinterpret_cast<double*>(work + q.ws + couple_norms_off(q.leaves));
     const int* tk = reinterpret_cast<const int*>(work + q.ws + couple_ints_off(q.leaves));
     if (pass == 2 && !tk[kCoupleReorth]) return;
-    if (q.vec)
-        cross_block<T, true>(q, it.y, blocks + (int64_t)it.y * 1024, norms + (int64_t)it.y * 32, lane);
-    else
-        cross_block<T, false>(q, it.y, blocks + (int64_t)it.y * 1024, norms + (int64_t)it.y * 32, lane);
+    const int S = 4 * batch_groups(q.leaves);
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+        if (q.mixed) {
+            if (q.vec)
+                cross_block_bf16<true>(q, 4 * it.y + wv, S, red[wv], red[wv] + 1024, lane);
+            else
+                cross_block_bf16<false>(q, 4 * it.y + wv, S, red[wv], red[wv] + 1024, lane);
+            done = true;
+        }
+    }
+    if (!done) {
+        if (q.vec)
+            cross_block<T, true>(q, 4 * it.y + wv, S, red[wv], red[wv] + 1024, lane);
+        else
+            cross_block<T, false>(q, 4 * it.y + wv, S, red[wv], red[wv] + 1024, lane);
+    }
+    __syncthreads();
+    double* out = blocks + (int64_t)it.y * 1024;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int idx = t + 256 * e;
+        out[idx] = ((red[0][idx] + red[1][idx]) + red[2][idx]) + red[3][idx];
+    }
+    if (t < 32)
+        norms[(int64_t)it.y * 32 + t] = ((red[0][1024 + t] + red[1][1024 + t]) + red[2][1024 + t]) + red[3][1024 + t];
 }
 
 // One workgroup per couple, after batch_cross: P = the sum of the row
@@ -601,7 +733,8 @@ __global__ __launch_bounds__(256) void batch_cross_sum(const BatchCouple* __rest
     const double* norms = reinterpret_cast<const double*>(work + q.ws + couple_norms_off(q.leaves));
     int* tk = reinterpret_cast<int*>(work + q.ws + couple_ints_off(q.leaves));
     if (pass == 2 && !tk[kCoupleReorth]) return;
-    sum_blocks_wg4(L, blocks, q.leaves);
+    const int nb = batch_groups(q.leaves);
+    sum_blocks_wg4(L, blocks, nb);
     if (threadIdx.x >= 64) return;
     double* P = blocks + (int64_t)q.leaves * 1024;
     double* Racc = blocks + (int64_t)(q.leaves + 1) * 1024;
@@ -621,7 +754,7 @@ __global__ __launch_bounds__(256) void batch_cross_sum(const BatchCouple* __rest
         const int j = lane & 31;
         double an = 0.0, pn = 0.0;
 #pragma unroll 1
-        for (int b = 0; b < q.leaves; ++b) an += norms[(int64_t)b * 32 + j];
+        for (int b = 0; b < nb; ++b) an += norms[(int64_t)b * 32 + j];
 #pragma unroll 4
         for (int i = 0; i < 32; ++i) pn = fmat(L.G[i][j], L.G[i][j], pn);
         const bool lost = j < q.n2 && !(2.0 * pn <= an);  // NaN counts as lost
@@ -630,7 +763,7 @@ __global__ __launch_bounds__(256) void batch_cross_sum(const BatchCouple* __rest
     }
 }
 
-// A2 rows of row block `blk` (the chunks blk, blk + S, ... of cross_block)
+// A2 rows of row block `blk` of S (the chunks blk, blk + S, ... of cross_block)
 // -= Q1 rows * P on the matrix cores in the working precision
 // (v_mfma_f32_16x16x4f32 for float, v_mfma_f64_16x16x4f64 for double),
 // accumulators initialised from A2 and B = -P, one rounding.
@@ -645,15 +778,13 @@ __global__ __launch_bounds__(256) void batch_cross_sum(const BatchCouple* __rest
 // sums in `sums`) straight from the accumulators: lane (li, lq) holds rows
 // 16 lq .. 16 lq + 15 of columns li and 16 + li of each 64-row chunk and
 // feeds rows 16 lq + 8 h + s to k-slot s of the h-th 32-row product, the
-// same assignment in every lane.  The result goes to G as the second
-// panel's Gram matrix of superleaf blk (the flat path only sums them).
+// same assignment in every lane.  The Gram slab is left in `sums`.
 template <typename T, bool VEC, bool GRAM = false>
-__device__ __forceinline__ void project_block(const BatchCouple& q, int blk, const double* __restrict__ P, int lane,
-                                              double* sums = nullptr, double* __restrict__ G = nullptr) {
+__device__ __forceinline__ void project_block(const BatchCouple& q, int blk, int S, const double* __restrict__ P,
+                                              int lane, double* sums = nullptr) {
     static_assert(!GRAM || sizeof(T) == 4, "the fused Gram is the mixed (FP32) method's");
     using M = Mfma16<T>;
     const int li = lane & 15, lq = lane >> 4;
-    const int S = q.leaves;
     const int chunks = (q.m + 63) / 64;
     const T* Q1 = reinterpret_cast<const T*>(q.Q1);
     T* A2 = reinterpret_cast<T*>(q.A2);
@@ -727,32 +858,37 @@ __device__ __forceinline__ void project_block(const BatchCouple& q, int blk, con
             g.template flush<2>(lane);
         }
     }
-    if constexpr (GRAM) g.store(G, lane);
 }
 
+// Item (couple, g) by four waves, row blocks as in batch_cross.  With q.gram
+// the four Gram slabs go, added in wave order, to the second panel's side
+// block g (the flat path only sums them).
 template <typename T>
-__global__ __launch_bounds__(64, 2) void batch_project(const BatchCouple* __restrict__ couples,
-                                                        const int2* __restrict__ items, char* __restrict__ work,
-                                                        int pass) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void batch_project(
+    const BatchCouple* __restrict__ couples, const int2* __restrict__ items, char* __restrict__ work, int pass) {
     const int2 it = items[blockIdx.x];
     const BatchCouple& q = couples[it.x];
+    const int wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     if (pass == 2 && !reinterpret_cast<const int*>(work + q.ws + couple_ints_off(q.leaves))[kCoupleReorth]) return;
     const double* P = reinterpret_cast<const double*>(work + q.ws) + (int64_t)q.leaves * 1024;
+    const int blk = 4 * it.y + wv, S = 4 * batch_groups(q.leaves);
     if constexpr (sizeof(T) == 4) {
         if (q.gram) {
-            __shared__ double sums[12 * 64];
-            double* G = reinterpret_cast<double*>(work + q.side2) + (int64_t)it.y * 1024;
+            __shared__ double sums[4][12 * 64];
             if (q.vec)
-                project_block<T, true, true>(q, it.y, P, threadIdx.x, sums, G);
+                project_block<T, true, true>(q, blk, S, P, lane, sums[wv]);
             else
-                project_block<T, false, true>(q, it.y, P, threadIdx.x, sums, G);
+                project_block<T, false, true>(q, blk, S, P, lane, sums[wv]);
+            __syncthreads();
+            gram32_combine(sums, reinterpret_cast<double*>(work + q.side2) + (int64_t)it.y * 1024);
             return;
         }
     }
     if (q.vec)
-        project_block<T, true>(q, it.y, P, threadIdx.x);
+        project_block<T, true>(q, blk, S, P, lane);
     else
-        project_block<T, false>(q, it.y, P, threadIdx.x);
+        project_block<T, false>(q, blk, S, P, lane);
 }
 
 }  // namespace tsqr

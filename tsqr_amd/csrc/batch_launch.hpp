@@ -23,6 +23,7 @@ struct ItemList {
 struct BatchStage {
     ItemList probs;              // (problem, 0) for every problem of the stage
     ItemList leaves;             // every superleaf
+    ItemList groups;             // (problem, g < batch_groups(superleaves)): the streaming passes
     ItemList level1;             // every level-1 tree block
     std::vector<ItemList> down;  // down[l]: every tile of level l (l >= 1)
     std::vector<ItemList> up;    // up[l]: every tile of level l >= 1, root included
@@ -36,7 +37,7 @@ struct BatchDev {
     int nprobs = 0;
     const BatchCouple* couples = nullptr;  // device
     int ncouples = 0;
-    ItemList couple_rows;  // (couple, row block) for every row block of every couple
+    ItemList couple_rows;  // (couple, g < batch_groups(row blocks)) for every couple
     ItemList couple_list;  // (couple, 0) for every couple
     BatchStage stage[2];  // first panels (and whole matrices with n <= 32), second panels
 };

@@ -19,7 +19,7 @@ void launch_panel_stage(const BatchDev& b, const BatchStage& s, char* work, hipS
     const int hh_grid = mixed ? (s.leaves.count < 1024 ? s.leaves.count : 1024) : s.leaves.count;
     if constexpr (sizeof(T) == 4) {
         if (mixed) {
-            hipLaunchKernelGGL(batch_mixed_gram<>, dim3(s.leaves.count), dim3(64), 0, st, b.probs, s.leaves.items,
+            hipLaunchKernelGGL(batch_mixed_gram<>, dim3(s.groups.count), dim3(256), 0, st, b.probs, s.groups.items,
                                work);
             hipLaunchKernelGGL(batch_mixed_flat<>, dim3(s.probs.count), dim3(256), 0, st, b.probs, s.probs.items,
                                work, b.gate, fallback);
@@ -46,7 +46,7 @@ void launch_panel_stage(const BatchDev& b, const BatchStage& s, char* work, hipS
     if constexpr (sizeof(T) == 4) {
         if (mixed)
             for (int pass = 1; pass <= 2; ++pass) {
-                hipLaunchKernelGGL(batch_mixed_q<>, dim3(s.leaves.count), dim3(64), 0, st, b.probs, s.leaves.items,
+                hipLaunchKernelGGL(batch_mixed_q<>, dim3(s.groups.count), dim3(256), 0, st, b.probs, s.groups.items,
                                    work, pass);
                 hipLaunchKernelGGL(batch_mixed_check<>, dim3(s.probs.count), dim3(256), 0, st, b.probs,
                                    s.probs.items, work, pass, b.tol);
@@ -72,11 +72,11 @@ int batch_launch(const BatchDev& b, void* work_v, hipStream_t st) {
     if (b.ncouples) {
         const int passes = b.method == 1 ? 2 : 1;
         for (int pass = 1; pass <= passes; ++pass) {
-            hipLaunchKernelGGL(batch_cross<T>, dim3(b.couple_rows.count), dim3(64), 0, st, b.couples,
+            hipLaunchKernelGGL(batch_cross<T>, dim3(b.couple_rows.count), dim3(256), 0, st, b.couples,
                                b.couple_rows.items, work, pass);
             hipLaunchKernelGGL(batch_cross_sum<T>, dim3(b.couple_list.count), dim3(256), 0, st, b.couples,
                                b.couple_list.items, work, pass);
-            hipLaunchKernelGGL(batch_project<T>, dim3(b.couple_rows.count), dim3(64), 0, st, b.couples,
+            hipLaunchKernelGGL(batch_project<T>, dim3(b.couple_rows.count), dim3(256), 0, st, b.couples,
                                b.couple_rows.items, work, pass);
         }
     }

@@ -111,17 +111,17 @@ __device__ __forceinline__ void load8(float (&x)[8], const float* __restrict__ b
     }
 }
 
-// Pass 1a of the batch: the BF16x3 Gram matrix of superleaf `blk` of S of
-// an m-row panel (one wave), written to G (FP64, row-major 32 x 32).  The
-// superleaves partition the panel's 64-row chunks by stride: superleaf j
-// takes chunks j, j + S, j + 2S, ..., so the S waves of a panel sweep it
-// together over a window of S chunks (the Grams are only ever summed).  Lane
+// Pass 1a of the batch: the BF16x3 Gram matrix of row block `blk` of S of
+// an m-row panel (one wave), left in the wave's FP64 slab `sums`.  The row
+// blocks partition the panel's 64-row chunks by stride: block j takes
+// chunks j, j + S, j + 2S, ..., so the S waves of a panel sweep it together
+// over a window of S chunks (the Grams are only ever summed).  Lane
 // l loads rows 8(l/16)..+7 of column l%16 (and 16 + l%16) of each 32-row
 // step; a chunk's two steps are loaded together and the next chunk is in
 // flight while the current one is multiplied.
 template <int NT, bool VEC>
-__device__ __forceinline__ void mixed_gram_body(int m, int n, const float* __restrict__ A, int lda,
-                                                double* __restrict__ G, double* sums, int blk, int S) {
+__device__ __forceinline__ void mixed_gram_body(int m, int n, const float* __restrict__ A, int lda, double* sums,
+                                                int blk, int S) {
     const int lane = threadIdx.x & 63;
     const int col = lane & 15;
     const int rq = 8 * (lane >> 4);
@@ -159,7 +159,26 @@ __device__ __forceinline__ void mixed_gram_body(int m, int n, const float* __res
                 if constexpr (NT == 2) c1[q][t] = p1[q][t];
             }
     }
-    g.store(G, lane);
+}
+
+// G (FP64 row-major 32 x 32, both triangles) = the sum of a workgroup's
+// four GramAcc32 slabs sums[0..3] in wave order, by its 256 threads (after
+// a barrier): thread t takes register t/64 of lane t%64 in every block.
+__device__ __forceinline__ void gram32_combine(const double (*sums)[12 * 64], double* __restrict__ G) {
+    const int t = threadIdx.x;
+    const int lane = t & 63, v = t >> 6;
+    const int col = lane & 15, i = 4 * (lane >> 4) + v;
+    double g00 = sums[0][v * 64 + lane], g01 = sums[0][(4 + v) * 64 + lane], g11 = sums[0][(8 + v) * 64 + lane];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+        g00 += sums[w][v * 64 + lane];
+        g01 += sums[w][(4 + v) * 64 + lane];
+        g11 += sums[w][(8 + v) * 64 + lane];
+    }
+    G[i * 32 + col] = g00;
+    G[i * 32 + 16 + col] = g01;
+    G[(16 + col) * 32 + i] = g01;
+    G[(16 + i) * 32 + 16 + col] = g11;
 }
 
 // End of a streaming launch that sums a Gram matrix with GramAcc32 (256
@@ -544,15 +563,15 @@ __device__ __forceinline__ void q_chunk_bf16(float* tw, const Split3 (&xb)[NT], 
     q_chunk_finish32<NT, VEC>(tw, q, base, lda, rows, n, r0, g, lane);
 }
 
-// Q pass over superleaf `blk` of S of an m-row FP32 panel by one wave (the
+// Q pass over row block `blk` of S of an m-row FP32 panel by one wave (the
 // batched path; the chunks j, j + S, ... as in mixed_gram_body), the next
-// chunk's loads in flight while the current one is multiplied; its Gram
-// matrix to Gout (FP64, row-major 32 x 32).  `bf16` picks the form; tw is
-// the wave's LDS tile.
+// chunk's loads in flight while the current one is multiplied; the Gram
+// matrix of the stored rows is left in the wave's FP64 slab `sums`.  `bf16`
+// picks the form; tw is the wave's LDS tile.
 template <int NT, bool VEC>
 __device__ __forceinline__ void mixed_q_leaf(int m, int n, float* __restrict__ A, int lda,
-                                             const double* __restrict__ X, double* __restrict__ Gout, double* sums,
-                                             float* tw, int blk, int S, bool bf16) {
+                                             const double* __restrict__ X, double* sums, float* tw, int blk, int S,
+                                             bool bf16) {
     const int lane = threadIdx.x & 63;
     const int chunks = (m + 63) / 64;
     QGram acc;
@@ -588,7 +607,6 @@ __device__ __forceinline__ void mixed_q_leaf(int m, int n, float* __restrict__ A
             q_chunk_f64<float, NT, VEC>(a, mb, nk, A + (int64_t)ch * 64, lda, m - 64 * ch, n, 0, acc, tw, lane);
         }
     }
-    acc.g.store(Gout, lane);
 }
 
 // Q pass of a single matrix, streamed by a persistent grid (as
