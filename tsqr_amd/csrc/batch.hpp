@@ -763,6 +763,107 @@ __global__ __launch_bounds__(256) void batch_cross_sum(const BatchCouple* __rest
     }
 }
 
+// FP32 form of project_block: the same products (v_mfma_f32_16x16x4f32,
+// output register v of (t, jt) is row 16(l/16) + 4v + t, column 16 jt + l%16),
+// with A2 moved through the wave's LDS tile `tw` (column c, row r at
+// c * kTrPitch + r): it is loaded and stored in 16-byte column runs (lane l:
+// rows 4(l%16) .. +3 of column 4p + l/16, 256 contiguous bytes per column and
+// instruction) and exchanged with the accumulator layout in LDS, where each
+// lane reads and writes the same positions.
+template <bool VEC, bool GRAM>
+__device__ __forceinline__ void project_block_f32(const BatchCouple& q, int blk, int S, const double* __restrict__ P,
+                                                  int lane, double* sums, float* tw) {
+    const int li = lane & 15, lq = lane >> 4;
+    const int chunks = (q.m + 63) / 64;
+    const float* Q1 = reinterpret_cast<const float*>(q.Q1);
+    float* A2 = reinterpret_cast<float*>(q.A2);
+    auto lim_of = [&](int ch) { return (int)(q.m - (int64_t)ch * 64 < 64 ? q.m - (int64_t)ch * 64 : 64); };
+    float mb[8][2];  // -P in the B-operand layout of k-group kg (k = 4 kg + l/16)
+#pragma unroll
+    for (int kg = 0; kg < 8; ++kg)
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) mb[kg][jt] = (float)(-P[(4 * kg + lq) * 32 + 16 * jt + li]);
+    float qa[8][4], ca[8][4];  // Q1 (A operand layout) and A2 (column runs) of a chunk
+    auto load_block = [&](int ch) {
+        const float* Qb = Q1 + (int64_t)ch * 64;
+        const float* Ab = A2 + (int64_t)ch * 64;
+        const int lim = lim_of(ch);
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            load_run<float, 4, VEC>(qa[p], Qb, q.lda, 4 * li, 4 * p + lq, lim, 32);
+            load_run<float, 4, VEC>(ca[p], Ab, q.lda, 4 * li, 4 * p + lq, lim, q.n2);
+        }
+    };
+    GramAcc32 g;
+    if constexpr (GRAM) {
+        g.sums = sums;
+        g.zero(lane);
+    }
+    if (blk < chunks) load_block(blk);
+#pragma unroll 1
+    for (int ch = blk; ch < chunks; ch += S) {
+        wave_sync();  // the previous chunk's reads of the tile are done
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) tw[(4 * p + lq) * kTrPitch + 4 * li + t] = ca[p][t];
+        wave_sync();
+        typedef float v4 __attribute__((ext_vector_type(4)));
+        v4 acc[4][2];
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[t][jt][v] = tw[(16 * jt + li) * kTrPitch + 16 * lq + 4 * v + t];
+        float x[8][4];
+#pragma unroll
+        for (int kg = 0; kg < 8; ++kg)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) x[kg][t] = qa[kg][t];
+        if (ch + S < chunks) load_block(ch + S);
+#pragma unroll
+        for (int kg = 0; kg < 8; ++kg)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int jt = 0; jt < 2; ++jt)
+                    acc[t][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[kg][t], mb[kg][jt], acc[t][jt], 0, 0, 0);
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) tw[(16 * jt + li) * kTrPitch + 16 * lq + 4 * v + t] = acc[t][jt][v];
+        wave_sync();
+        const int lim = lim_of(ch);
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const int c = 4 * p + lq;
+            if (c < q.n2) {
+                float o[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) o[t] = tw[c * kTrPitch + 4 * li + t];
+                store_column<float, 4, VEC>(o, A2 + (int64_t)ch * 64 + (int64_t)c * q.lda + 4 * li, lim, 4 * li);
+            }
+        }
+        if constexpr (GRAM) {
+            // rows past the end and columns past n2 are zero in acc
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                float x0[8], x1[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    x0[e] = acc[e & 3][0][2 * h + (e >> 2)];
+                    x1[e] = acc[e & 3][1][2 * h + (e >> 2)];
+                }
+                g.template add<2>(x0, x1);
+            }
+            g.template flush<2>(lane);
+        }
+    }
+}
+
 // A2 rows of row block `blk` of S (the chunks blk, blk + S, ... of cross_block)
 // -= Q1 rows * P on the matrix cores in the working precision
 // (v_mfma_f32_16x16x4f32 for float, v_mfma_f64_16x16x4f64 for double),
@@ -781,8 +882,12 @@ __global__ __launch_bounds__(256) void batch_cross_sum(const BatchCouple* __rest
 // same assignment in every lane.  The Gram slab is left in `sums`.
 template <typename T, bool VEC, bool GRAM = false>
 __device__ __forceinline__ void project_block(const BatchCouple& q, int blk, int S, const double* __restrict__ P,
-                                              int lane, double* sums = nullptr) {
+                                              int lane, double* sums = nullptr, float* tw = nullptr) {
     static_assert(!GRAM || sizeof(T) == 4, "the fused Gram is the mixed (FP32) method's");
+    if constexpr (sizeof(T) == 4) {
+        project_block_f32<VEC, GRAM>(q, blk, S, P, lane, sums, tw);
+        return;
+    }
     using M = Mfma16<T>;
     const int li = lane & 15, lq = lane >> 4;
     const int chunks = (q.m + 63) / 64;
@@ -873,22 +978,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     if (pass == 2 && !reinterpret_cast<const int*>(work + q.ws + couple_ints_off(q.leaves))[kCoupleReorth]) return;
     const double* P = reinterpret_cast<const double*>(work + q.ws) + (int64_t)q.leaves * 1024;
     const int blk = 4 * it.y + wv, S = 4 * batch_groups(q.leaves);
+    __shared__ float tr[sizeof(T) == 4 ? 4 : 1][32 * kTrPitch];
+    float* tw = tr[sizeof(T) == 4 ? wv : 0];
     if constexpr (sizeof(T) == 4) {
         if (q.gram) {
             __shared__ double sums[4][12 * 64];
             if (q.vec)
-                project_block<T, true, true>(q, blk, S, P, lane, sums[wv]);
+                project_block<T, true, true>(q, blk, S, P, lane, sums[wv], tw);
             else
-                project_block<T, false, true>(q, blk, S, P, lane, sums[wv]);
+                project_block<T, false, true>(q, blk, S, P, lane, sums[wv], tw);
             __syncthreads();
             gram32_combine(sums, reinterpret_cast<double*>(work + q.side2) + (int64_t)it.y * 1024);
             return;
         }
     }
     if (q.vec)
-        project_block<T, true>(q, blk, S, P, lane);
+        project_block<T, true>(q, blk, S, P, lane, nullptr, tw);
     else
-        project_block<T, false>(q, blk, S, P, lane);
+        project_block<T, false>(q, blk, S, P, lane, nullptr, tw);
 }
 
 }  // namespace tsqr
