@@ -463,8 +463,19 @@ def main():
         stats = (0, 0, mixed["flat"])
         rows = m if mixed["flat"] else 0
         blocks = 3 if n > 16 else 1
-        f64 = {"leaf_q": rows / 64 * 48 * 2048 * mixed["passes"]}
-        bf16 = {"leaf_r": rows / 32 * 6 * blocks * 16384, "leaf_q": rows / 32 * 6 * blocks * 16384 * mixed["passes"]}
+        nt = 2 if n > 16 else 1
+        # per pass: the Gram of the stored Q (BF16x3, 6 MFMAs of 16x16x32 per 32-row step and block), and Q = A X
+        # either as BF16x3 products (4 row tiles x nt column tiles x 6 MFMAs per 64 rows) or on
+        # v_mfma_f64_16x16x4f64 (48 per 64 rows at n = 32: X is upper triangular)
+        q_f64 = q_bf16 = 0.0
+        for p_ in range(mixed["passes"]):
+            if mixed["bf16_q"][p_]:
+                q_bf16 += rows / 64 * 4 * nt * 6 * 16384
+            else:
+                q_f64 += rows / 64 * 48 * 2048
+        f64 = {"leaf_q": q_f64}
+        bf16 = {"leaf_r": rows / 32 * 6 * blocks * 16384,
+                "leaf_q": rows / 32 * 6 * blocks * 16384 * mixed["passes"] + q_bf16}
     else:
         stats, f64, bf16 = (0, 0, 0), {}, {}
     for name in kern:

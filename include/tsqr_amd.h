@@ -155,6 +155,14 @@ int tsqr_set_mixed(double gate, double tol);
  * ||Q^T Q - I||_F estimate.  Synchronises the device.  Diagnostic. */
 int tsqr_mixed_stats(const void *work, int m, int n, int *flat, int *passes, int *check, double *residual);
 
+/* After a tsqr_mixed_s* call on the same stream, with its workspace: which
+ * form the Q passes took (no counterpart in the reference).  *bf16_pass1 = 1
+ * if pass 1 formed Q1 = A R1^{-1} with BF16x3 products on the BF16 matrix
+ * cores (R1 well conditioned: kappa_est <= 2n), 0 if with FP64 products;
+ * *bf16_pass2 likewise for the second pass (0 when it did not run).
+ * Synchronises the device.  Diagnostic. */
+int tsqr_mixed_q_forms(const void *work, int m, int n, int *bf16_pass1, int *bf16_pass2);
+
 /* Batched TSQR of many independent matrices of uneven shape (no counterpart
  * in the reference, which factors one matrix per call).  Matrix i is m[i] x
  * n[i] (1 <= n[i] <= 64, m[i] >= n[i]) column-major at the device pointer
@@ -192,6 +200,10 @@ int tsqr_batch_set_mixed(tsqr_batch *plan, double gate, double tol);
  * panels whose final residual check failed. */
 int tsqr_batch_stats(const tsqr_batch *plan, const void *work, int *cholesky, int *householder, int *second_pass,
                      int *check_failed);
+/* After a mixed run (synchronises the device): panels whose first Q pass ran
+ * on the BF16 matrix cores (BF16x3 products), and panels whose second pass
+ * did. */
+int tsqr_batch_q_forms(const tsqr_batch *plan, const void *work, int *bf16_pass1, int *bf16_pass2);
 int tsqr_batch_destroy(tsqr_batch *plan);
 
 /* Chooses how level-0 superleaves (up to 8 Householder tiles of rows each)

@@ -73,8 +73,12 @@ def test_batch_uneven_shapes(method, dtype):
     if method == "mixed":
         assert stats["check_failed"] == 0
         assert stats["cholesky"] == panels  # well-conditioned inputs stay on the CholeskyQR path
+        # iid N(0,1) panels at least 20n tall: kappa_est <= 2n, so Q is formed on the BF16 matrix cores
+        tall = sum(1 + (n > 32) for m, n in SHAPES if m >= 20 * min(n, 32))
+        assert stats["bf16_q_pass1"] >= tall
     else:
         assert stats["householder"] == panels
+        assert stats["bf16_q_pass1"] == 0
 
 
 def test_batch_mixed_ill_conditioned_member_falls_back():

@@ -67,6 +67,10 @@ def test_mixed_shapes(m, n):
     # the library's residual estimate tracks the measured orthogonality
     assert LAST_MIXED_STATS["residual"] <= 8e-7
     assert abs(LAST_MIXED_STATS["residual"] - mt["orth"]) <= 0.5 * mt["orth"] + 2e-7
+    if m >= 20 * n:
+        # iid N(0,1), tall: kappa_est = ||R||_F ||R^-1||_F is within 2n, so Q1 = A R1^-1 is formed with
+        # BF16x3 products on the BF16 matrix cores
+        assert LAST_MIXED_STATS["bf16_q"][0] == 1
 
 
 def test_mixed_single_tile_is_householder():
@@ -94,6 +98,9 @@ def test_mixed_conditioned_inputs_stay_on_cholesky_path(cond):
     assert st == 0
     check_mixed(A, Q, R, f"mixed cond={cond:g}")
     assert LAST_MIXED_STATS["flat"] == 1 and LAST_MIXED_STATS["check"] == 0
+    if cond >= 1e2:
+        # kappa_est above 2n: FP64 products form Q1 (the BF16x3 sums' error grows with kappa(R1^-1))
+        assert LAST_MIXED_STATS["bf16_q"][0] == 0
 
 
 @pytest.mark.parametrize("cond", [1e4, 1e6])
@@ -122,7 +129,25 @@ def test_mixed_second_pass():
     assert st == 0
     assert LAST_MIXED_STATS["flat"] == 1 and LAST_MIXED_STATS["passes"] == 2
     assert LAST_MIXED_STATS["check"] == 1
+    # pass 1 on FP64 products (cond 1e2), pass 2 on BF16x3 (R2^-1 is close to the identity)
+    assert LAST_MIXED_STATS["bf16_q"] == [0, 1]
     check_mixed(A, Q, R, "mixed forced second pass")
+
+
+def test_mixed_second_pass_bf16_both():
+    """Both Q passes on the BF16 matrix cores (iid input, forced second
+    pass): the result meets the same bounds."""
+    from tsqr_amd._lib import set_mixed
+
+    A = O.gen_normal(120000, 32, np.float32, seed=17)
+    set_mixed(1024.0, 1e-12)
+    try:
+        st, Q, R, _ = call("tsqr_mixed", A)
+    finally:
+        set_mixed(1024.0, 8e-7)
+    assert st == 0
+    assert LAST_MIXED_STATS["passes"] == 2 and LAST_MIXED_STATS["bf16_q"] == [1, 1]
+    check_mixed(A, Q, R, "mixed bf16 both passes")
 
 
 def test_mixed_deterministic():

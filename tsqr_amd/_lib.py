@@ -40,6 +40,7 @@ EXPORTS = (
     "tsqr_mixed_s_stream",
     "tsqr_set_mixed",
     "tsqr_mixed_stats",
+    "tsqr_mixed_q_forms",
     "qr_mixed_s",
     "qr_mixed_s_stream",
     "tsqr_set_leaf_tiles",
@@ -54,6 +55,7 @@ EXPORTS = (
     "tsqr_batch_graph_launch",
     "tsqr_batch_set_mixed",
     "tsqr_batch_stats",
+    "tsqr_batch_q_forms",
     "tsqr_batch_destroy",
     "tsqr_check_s",
     "tsqr_check_d",
@@ -115,6 +117,8 @@ def lib() -> ctypes.CDLL:
     pi = ctypes.POINTER(i32)
     L.tsqr_mixed_stats.argtypes = [vp, i32, i32, pi, pi, pi, ctypes.POINTER(ctypes.c_double)]
     L.tsqr_mixed_stats.restype = i32
+    L.tsqr_mixed_q_forms.argtypes = [vp, i32, i32, pi, pi]
+    L.tsqr_mixed_q_forms.restype = i32
     ip = ctypes.POINTER(i32)
     L.tsqr_batch_create.argtypes = [i32, i32, i32, ip, ip, ctypes.POINTER(vp), ip, ctypes.POINTER(vp), ip,
                                     ctypes.POINTER(vp)]
@@ -129,6 +133,8 @@ def lib() -> ctypes.CDLL:
     L.tsqr_batch_set_mixed.restype = i32
     L.tsqr_batch_stats.argtypes = [vp, vp, ip, ip, ip, ip]
     L.tsqr_batch_stats.restype = i32
+    L.tsqr_batch_q_forms.argtypes = [vp, vp, ip, ip]
+    L.tsqr_batch_q_forms.restype = i32
     L.tsqr_batch_destroy.argtypes = [vp]
     L.tsqr_batch_destroy.restype = i32
     L.tsqr_set_leaf_tiles.argtypes = [i32, i32, i32, i32]
@@ -216,7 +222,12 @@ def mixed_stats(work_ptr: int, m: int, n: int) -> dict:
     st = lib().tsqr_mixed_stats(work_ptr, m, n, ctypes.byref(f), ctypes.byref(p), ctypes.byref(c), ctypes.byref(r))
     if st != TSQR_OK:
         raise TsqrError(st, "tsqr_mixed_stats")
-    return {"flat": f.value, "passes": p.value, "check": c.value, "residual": r.value}
+    b1, b2 = ctypes.c_int(0), ctypes.c_int(0)
+    st = lib().tsqr_mixed_q_forms(work_ptr, m, n, ctypes.byref(b1), ctypes.byref(b2))
+    if st != TSQR_OK:
+        raise TsqrError(st, "tsqr_mixed_q_forms")
+    return {"flat": f.value, "passes": p.value, "check": c.value, "residual": r.value,
+            "bf16_q": [b1.value, b2.value]}
 
 
 def status_string(status: int) -> str:

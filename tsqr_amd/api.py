@@ -167,7 +167,13 @@ class Batch:
         st = lib().tsqr_batch_stats(self._h, self.work.data_ptr(), *[ctypes.byref(x) for x in v])
         if st != 0:
             raise TsqrError(st, "tsqr_batch_stats")
-        return dict(zip(("cholesky", "householder", "second_pass", "check_failed"), (x.value for x in v)))
+        out = dict(zip(("cholesky", "householder", "second_pass", "check_failed"), (x.value for x in v)))
+        f = [ctypes.c_int(0) for _ in range(2)]
+        st = lib().tsqr_batch_q_forms(self._h, self.work.data_ptr(), *[ctypes.byref(x) for x in f])
+        if st != 0:
+            raise TsqrError(st, "tsqr_batch_q_forms")
+        out["bf16_q_pass1"], out["bf16_q_pass2"] = f[0].value, f[1].value
+        return out
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

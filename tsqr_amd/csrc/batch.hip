@@ -318,6 +318,23 @@ int tsqr_batch_stats(const tsqr_batch* b, const void* work, int* cholesky, int* 
     return TSQR_OK;
 }
 
+int tsqr_batch_q_forms(const tsqr_batch* b, const void* work, int* bf16_pass1, int* bf16_pass2) {
+    if (!b || !work || !bf16_pass1 || !bf16_pass2) return TSQR_ERR_ARG;
+    *bf16_pass1 = *bf16_pass2 = 0;
+    if (hipDeviceSynchronize() != hipSuccess) return TSQR_ERR_HIP;
+    for (const BatchProb& p : b->probs) {
+        int ctl[5];
+        if (hipMemcpy(ctl, reinterpret_cast<const char*>(work) + p.ws + p.w.ctl, sizeof(ctl),
+                      hipMemcpyDeviceToHost) != hipSuccess)
+            return TSQR_ERR_HIP;
+        if (p.mixed && ctl[0]) {
+            *bf16_pass1 += ctl[kCtlBf16Q1] ? 1 : 0;
+            *bf16_pass2 += (ctl[1] && ctl[kCtlBf16Q2]) ? 1 : 0;
+        }
+    }
+    return TSQR_OK;
+}
+
 int tsqr_batch_destroy(tsqr_batch* b) {
     if (!b) return TSQR_OK;
     release_graph(b);

@@ -899,6 +899,26 @@ int tsqr_mixed_stats(const void* work, int m, int n, int* flat, int* passes, int
     return TSQR_OK;
 }
 
+int tsqr_mixed_q_forms(const void* work, int m, int n, int* bf16_pass1, int* bf16_pass2) {
+    if (!work || !bf16_pass1 || !bf16_pass2) return TSQR_ERR_ARG;
+    if (n < 1 || m < n || n > 32) return TSQR_ERR_SHAPE;
+    TreeDesc d;
+    if (!plan_tsqr<float>(m, n, &d, mixed_opts().path)) return TSQR_ERR_SHAPE;
+    *bf16_pass1 = *bf16_pass2 = 0;
+    if (d.lv[0].rows <= d.H) return TSQR_OK;  // single tile: Householder
+    const WsLayout w = ws_layout(d, sizeof(float));
+    int ctl[5];
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(ctl, reinterpret_cast<const char*>(work) + w.ctl, sizeof(ctl), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return TSQR_ERR_HIP;
+    if (ctl[0]) {
+        *bf16_pass1 = ctl[kCtlBf16Q1] ? 1 : 0;
+        *bf16_pass2 = (ctl[1] && ctl[kCtlBf16Q2]) ? 1 : 0;
+    }
+    return TSQR_OK;
+}
+
 int tsqr_set_leaf_tiles(int dtype, int n, int log2m, int tiles) {
     if ((dtype != TSQR_FLOAT && dtype != TSQR_DOUBLE) || n < 1 || n > 32 || log2m < 0 || log2m > 31 || tiles < 0 ||
         tiles > 8)
