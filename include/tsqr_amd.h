@@ -145,7 +145,10 @@ int qr_mixed_s_stream(int m, int n, float *A, int lda, float *R, int ldr, void *
                       void *stream);
 
 /* Gate (>= 1, default 1024) and residual tolerance (> 0, default 8e-7) of
- * the mixed entry points, process-wide. */
+ * the mixed entry points, process-wide.  The gate is the largest
+ * ||R||_F ||R^{-1}||_F the BF16x3 Gram pass accepts; above it the FP64 Gram
+ * pass accepts up to max(gate, 1e7), or up to the gate itself when the gate
+ * is below its default (a gate of 1 forces the Householder tree). */
 int tsqr_set_mixed(double gate, double tol);
 
 /* After a tsqr_mixed_s* call on the same stream, with its workspace: *flat =

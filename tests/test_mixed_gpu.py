@@ -104,14 +104,29 @@ def test_mixed_conditioned_inputs_stay_on_cholesky_path(cond):
 
 
 @pytest.mark.parametrize("cond", [1e4, 1e6])
-def test_mixed_ill_conditioned_falls_back(cond):
-    """kappa_est above the gate: the Householder tree runs, with its
-    kappa-independent bounds."""
+def test_mixed_ill_conditioned_escalates_to_fp64_gram(cond):
+    """kappa_est above the BF16x3 gate: pass 1 is redone on the FP64 Gram
+    (exact products) and Q1 is formed with FP64 products.  Q1 is then
+    orthonormal to about u64 kappa^2: enough at cond 1e4, while at cond 1e6
+    the second pass (BF16x3, R2^-1 close to the identity) restores it."""
     A = O.gen_conditioned(60000, 32, cond, np.float32, seed=12)
     st, Q, R, _ = call("tsqr_mixed", A)
     assert st == 0
+    assert LAST_MIXED_STATS["flat"] == 1 and LAST_MIXED_STATS["bf16_q"][0] == 0
+    assert LAST_MIXED_STATS["passes"] == (2 if cond >= 1e6 else 1)
+    if LAST_MIXED_STATS["passes"] == 2:
+        assert LAST_MIXED_STATS["bf16_q"][1] == 1
+    check_mixed(A, Q, R, f"mixed cond={cond:g} FP64 Gram")
+
+
+def test_mixed_beyond_fp64_gate_falls_back():
+    """Numerically rank deficient in FP32 (cond 1e9): both gates reject,
+    the Householder tree runs, with its kappa-independent bounds."""
+    A = O.gen_conditioned(60000, 32, 1e9, np.float32, seed=12)
+    st, Q, R, _ = call("tsqr_mixed", A)
+    assert st == 0
     assert LAST_MIXED_STATS["flat"] == 0
-    check_mixed(A, Q, R, f"mixed cond={cond:g} fallback", orth_bound=8 * 32 * U32)
+    check_mixed(A, Q, R, "mixed cond=1e9 fallback", orth_bound=8 * 32 * U32)
 
 
 def test_mixed_second_pass():

@@ -656,8 +656,9 @@ __global__ __launch_bounds__(kSumThreads) void tsqr_gram_flat(const TreeDesc d, 
                                                         const double* __restrict__ partial, T* __restrict__ R,
                                                         int ldr, double* __restrict__ side, int* __restrict__ ctl,
                                                         double gate, double* __restrict__ rkeep,
-                                                        int bf16q = 0) {
+                                                        int bf16q = 0, const int* __restrict__ skip = nullptr) {
     __shared__ GramLds L;
+    if (skip && skip[0]) return;  // an earlier pass already took the flat path
     const int t = threadIdx.x;
     sum_partials(L, nparts, partial, t);
     __syncthreads();
@@ -826,9 +827,11 @@ __device__ __forceinline__ void stream_reduce(GramLds& L, const v4d& a00, const 
 // kCtlStreamTickets ticket words, zero on entry.
 template <typename T, int NT, bool VEC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void tsqr_gram_stream(const TreeDesc d, const T* __restrict__ A, int lda,
-                                                         double* __restrict__ partial, int* __restrict__ tk) {
+                                                         double* __restrict__ partial, int* __restrict__ tk,
+                                                         const int* __restrict__ skip = nullptr) {
     __shared__ GramLds L;
     __shared__ int bcast;
+    if (skip && skip[0]) return;  // an earlier pass already took the flat path
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wv = t >> 6;

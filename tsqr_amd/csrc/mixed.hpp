@@ -29,9 +29,12 @@
 //           BF16x3 where e1 <= kBf16QMaxResidual, with G3 = Q^T Q fused
 //           again), R = R2 R1 in FP64; e2 = ||G3 - I||_F is recorded and
 //           status 1 is set if e2 > tol.
-// If the first Cholesky breaks down or kappa_est exceeds the gate, the whole
-// matrix takes the Householder tree instead (the superleaf and tree kernels,
-// driven by the same ctl word), so no input is rejected.
+// If the first Cholesky breaks down or kappa_est exceeds the gate, pass 1
+// is redone on the FP64 Gram of the AUTO path (exact products: the flat
+// R1 holds to kappa_est <= kMixedGram64Gate, with Q1 formed from FP64
+// products and the second pass restoring the orthogonality).  Only where
+// that fails too does the whole matrix take the Householder tree (the leaf
+// and tree kernels, driven by the same ctl word), so no input is rejected.
 //
 // ctl words (int): ctl[0] = 1 flat Gram taken, ctl[1] = 1 second pass
 // requested, ctl[2] = status (0 ok, 1 residual above tol after the second

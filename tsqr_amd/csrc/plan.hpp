@@ -127,6 +127,18 @@ constexpr int kCtlStreamTickets4 = 256; //   two streamed Q passes
 constexpr int kCtlBf16Q1 = 3;
 constexpr int kCtlBf16Q2 = 4;
 constexpr double kBf16QMaxResidual = 0.5;
+// Gate of the FP32 mixed path's FP64-Gram escalation (taken where the
+// BF16x3 Gram's gate rejects): the FP64 Gram of FP32 data has exact
+// products, so R1 is accurate to about u64 kappa^2 relative, and Q1 = A
+// R1^{-1} is orthonormal to O(u64 kappa_est^2) <= 1e-2 below the gate,
+// which the second CholeskyQR pass restores to FP32 level.  A caller's gate
+// below the default kMixedGate (tsqr_set_mixed, tsqr_set_leaf_path) limits
+// this pass too, so a gate of 1 still forces the Householder fallback.
+constexpr double kMixedGate = 1024.0;
+constexpr double kMixedGram64Gate = 1e7;
+inline double mixed_gram64_gate(double gate) {
+    return gate < kMixedGate ? gate : (gate > kMixedGram64Gate ? gate : kMixedGram64Gate);
+}
 
 // The streaming Gram kernel (tsqr_gram_stream, gram.hpp) runs P workgroups
 // of 4 waves, P = stream_groups(m): at most kStreamMaxGroups, and four

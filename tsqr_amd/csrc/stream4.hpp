@@ -73,10 +73,11 @@ __device__ __forceinline__ f4v load_rows4_tail(const float* __restrict__ A, int6
 template <int NC, bool VEC, bool FULL>
 __global__ __launch_bounds__(256) void tsqr_gram_stream4(int m, int n, const float* __restrict__ A, int lda,
                                                          double* __restrict__ partial, int* __restrict__ tk,
-                                                         int cached) {
+                                                         int cached, const int* __restrict__ skip = nullptr) {
     constexpr int NP = NC * (NC + 1) / 2;
     __shared__ GramLds L;
     __shared__ int bcast;
+    if (skip && skip[0]) return;  // an earlier pass already took the flat path
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wv = t >> 6;

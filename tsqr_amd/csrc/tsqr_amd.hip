@@ -163,7 +163,7 @@ constexpr double kGateF64 = 1e5;
 std::mutex g_opts_mu;
 LaunchOpts g_opts_f32{kLeafAuto, 1024.0, 0.0};
 LaunchOpts g_opts_f64{kLeafCholQR2, kGateF64, kTolF64};
-LaunchOpts g_opts_mixed{kLeafCholQR2, 1024.0, 8e-7};
+LaunchOpts g_opts_mixed{kLeafCholQR2, kMixedGate, 8e-7};
 
 template <typename T>
 LaunchOpts current_opts() {

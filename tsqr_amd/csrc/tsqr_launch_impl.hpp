@@ -124,6 +124,28 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
                                partial, stream_tk);
             hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
                                (const double*)stream_sum, R, ldr, side, ctl, o.gate, rkeep, 1);
+            // Escalation where the BF16x3 gate rejected the matrix: the FP64
+            // Gram of the AUTO path (exact products) with the CholeskyQR2
+            // gate mixed_gram64_gate; both kernels return at once when the
+            // first pass succeeded (ctl[0]).  Pass 1 then forms Q1 with FP64
+            // products and pass 2 restores the orthogonality.
+            int* tk64 = ctl + kCtlStreamTickets2;
+            const bool four = use_stream4(d.m);
+            const int g64 = four ? stream_groups4(d.m) : sgroups;
+            if (four) {
+                if (d.n == N)
+                    hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, true>), dim3(g64), dim3(256), 0, st, d.m, d.n,
+                                       (const float*)A, lda, partial, tk64, kCachedChunks, (const int*)ctl);
+                else
+                    hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, false>), dim3(g64), dim3(256), 0, st, d.m, d.n,
+                                       (const float*)A, lda, partial, tk64, kCachedChunks, (const int*)ctl);
+            } else {
+                hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(g64), dim3(256), 0, st, d, (const T*)A, lda,
+                                   partial, tk64, (const int*)ctl);
+            }
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, stream_group_count(g64),
+                               (const double*)(partial + (size_t)g64 * 1024), R, ldr, side, ctl,
+                               mixed_gram64_gate(o.gate), rkeep, 0, (const int*)ctl);
         } else {
             hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
                                partial, stream_tk);
