@@ -116,6 +116,55 @@ __device__ __forceinline__ void store_four(const Four<T>& o, T* __restrict__ X, 
     }
 }
 
+// A 64 x 64 FP32 block of C (rows r0.., columns j0..) moves between HBM and
+// the accumulator layout of the update kernels through the wave's LDS tile
+// `tw` (column c, row r at c * kCTilePitch + r): HBM sees 16-byte column runs
+// (lane l: rows 4(l%16) .. +3 of column 4p + l/16, p = 0..15, 256 contiguous
+// bytes per column and instruction); the accumulator register v of (t, jt)
+// holds row 16(l/16) + 4v + t, column 16 jt + l%16, and each lane reads and
+// writes the same tile positions.
+constexpr int kCTilePitch = 66;  // floats per LDS column of the tile
+constexpr int kCTileFloats = 64 * kCTilePitch;
+
+template <bool VEC>
+__device__ __forceinline__ void c_tile_load(f4v (&acc)[4][4], float* tw, const float* __restrict__ C, int64_t ldc,
+                                            int64_t r0, int j0, int64_t m, int N, int lane) {
+    const int li = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const Four<float> c = load_four<float, VEC>(C, ldc, r0 + 4 * li, j0 + 4 * p + lq, m, N);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) tw[(4 * p + lq) * kCTilePitch + 4 * li + t] = c.v[t];
+    }
+    wave_sync();
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t][jt][v] = tw[(16 * jt + li) * kCTilePitch + 16 * lq + 4 * v + t];
+}
+
+template <bool VEC>
+__device__ __forceinline__ void c_tile_store(const f4v (&acc)[4][4], float* tw, float* C, int64_t ldc, int64_t r0,
+                                             int j0, int64_t m, int N, int lane) {
+    const int li = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) tw[(16 * jt + li) * kCTilePitch + 16 * lq + 4 * v + t] = acc[t][jt][v];
+    wave_sync();
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        Four<float> c;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) c.v[t] = tw[(4 * p + lq) * kCTilePitch + 4 * li + t];
+        store_four<float, VEC>(c, C, ldc, r0 + 4 * li, j0 + 4 * p + lq, m, N);
+    }
+}
+
 // part[chunk] (n1 x n2, ld n1) for the 64 x 64 output tile (blockIdx.y,
 // blockIdx.z) = Q[rows of chunk, cols i]^T A[rows of chunk, cols j].
 template <typename T, bool VEC>
@@ -231,20 +280,25 @@ __global__ __launch_bounds__(64) void gemm_nn_mfma(int m, int K, int N, const T*
                                                    const T* __restrict__ B, int64_t ldb, T* __restrict__ C,
                                                    int64_t ldc) {
     using M = Mfma16<T>;
+    __shared__ float tile[sizeof(T) == 4 ? kCTileFloats : 1];
     const int lane = threadIdx.x;
     const int li = lane & 15, lq = lane >> 4;
     const int64_t r0 = (int64_t)blockIdx.x * 64;
     const int j0 = blockIdx.y * 64;
     // acc[t][jt] register v: row r0 + 4*row(v, lq) + t, column j0 + 16 jt + li
     typename M::v4 acc[4][4];
+    if constexpr (sizeof(T) == 4) {
+        c_tile_load<VEC>(acc, tile, C, ldc, r0, j0, m, N, lane);  // row(v, lq) = 4 lq + v
+    } else {
 #pragma unroll
-    for (int jt = 0; jt < 4; ++jt)
+        for (int jt = 0; jt < 4; ++jt)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const Four<T> c = load_four<T, VEC>(C, ldc, r0 + 4 * M::row(v, lq), j0 + 16 * jt + li, m, N);
+            for (int v = 0; v < 4; ++v) {
+                const Four<T> c = load_four<T, VEC>(C, ldc, r0 + 4 * M::row(v, lq), j0 + 16 * jt + li, m, N);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t][jt][v] = c.v[t];
-        }
+                for (int t = 0; t < 4; ++t) acc[t][jt][v] = c.v[t];
+            }
+    }
 #pragma unroll 1
     for (int k0 = 0; k0 < K; k0 += 16) {
         // A operand: lane feeds row 4*li + t of column k0 + 4*kg + lq
@@ -267,15 +321,19 @@ __global__ __launch_bounds__(64) void gemm_nn_mfma(int m, int K, int N, const T*
 #pragma unroll
                 for (int jt = 0; jt < 4; ++jt) acc[t][jt] = M::mma(xa[kg].v[t], bb[kg][jt], acc[t][jt]);
     }
+    if constexpr (sizeof(T) == 4) {
+        c_tile_store<VEC>(acc, tile, C, ldc, r0, j0, m, N, lane);
+    } else {
 #pragma unroll
-    for (int jt = 0; jt < 4; ++jt)
+        for (int jt = 0; jt < 4; ++jt)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            Four<T> c;
+            for (int v = 0; v < 4; ++v) {
+                Four<T> c;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) c.v[t] = acc[t][jt][v];
-            store_four<T, VEC>(c, C, ldc, r0 + 4 * M::row(v, lq), j0 + 16 * jt + li, m, N);
-        }
+                for (int t = 0; t < 4; ++t) c.v[t] = acc[t][jt][v];
+                store_four<T, VEC>(c, C, ldc, r0 + 4 * M::row(v, lq), j0 + 16 * jt + li, m, N);
+            }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -404,18 +462,11 @@ __global__ __launch_bounds__(64) void gemm_tn_bf16x3(int m, int n1, int n2, cons
 // before its stores.
 template <bool VEC>
 __device__ __forceinline__ void nn_bf16x3_block(int m, int K, int N, const float* X, int64_t ldx, const float* B,
-                                                int64_t ldb, float* C, int64_t ldc, int64_t r0, int j0) {
+                                                int64_t ldb, float* C, int64_t ldc, int64_t r0, int j0, float* tw) {
     const int lane = threadIdx.x;
     const int li = lane & 15, lq = lane >> 4;
     f4v acc[4][4];  // [t][jt]
-#pragma unroll
-    for (int jt = 0; jt < 4; ++jt)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const Four<float> c = load_four<float, VEC>(C, ldc, r0 + 16 * lq + 4 * v, j0 + 16 * jt + li, m, N);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t][jt][v] = c.v[t];
-        }
+    c_tile_load<VEC>(acc, tw, C, ldc, r0, j0, m, N, lane);
 #pragma unroll 1
     for (int k0 = 0; k0 < K; k0 += 32) {
         Split3 sx[4], sb[4];
@@ -446,22 +497,15 @@ __device__ __forceinline__ void nn_bf16x3_block(int m, int K, int N, const float
 #pragma unroll
             for (int jt = 0; jt < 4; ++jt) acc[t][jt] = mma_bf16x3(sx[t], sb[jt], acc[t][jt]);
     }
-#pragma unroll
-    for (int jt = 0; jt < 4; ++jt)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            Four<float> c;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) c.v[t] = acc[t][jt][v];
-            store_four<float, VEC>(c, C, ldc, r0 + 16 * lq + 4 * v, j0 + 16 * jt + li, m, N);
-        }
+    c_tile_store<VEC>(acc, tw, C, ldc, r0, j0, m, N, lane);
 }
 
 template <bool VEC>
 __global__ __launch_bounds__(64) void gemm_nn_bf16x3(int m, int K, int N, const float* __restrict__ X, int64_t ldx,
                                                      const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
                                                      int64_t ldc) {
-    nn_bf16x3_block<VEC>(m, K, N, X, ldx, B, ldb, C, ldc, (int64_t)blockIdx.x * 64, blockIdx.y * 64);
+    __shared__ float tile[kCTileFloats];
+    nn_bf16x3_block<VEC>(m, K, N, X, ldx, B, ldb, C, ldc, (int64_t)blockIdx.x * 64, blockIdx.y * 64, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -598,9 +642,10 @@ template <bool VEC>
 __global__ __launch_bounds__(64) void reorth_apply(int m, int K, int N, float* Q, int64_t ldq, int c0,
                                                    const float* __restrict__ Dneg, int64_t ldd,
                                                    const int* __restrict__ st) {
+    __shared__ float tile[kCTileFloats];
     if (!st[0]) return;
     nn_bf16x3_block<VEC>(m, K, N, Q, ldq, Dneg + (int64_t)c0 * ldd, ldd, Q + (int64_t)c0 * ldq, ldq,
-                         (int64_t)blockIdx.x * 64, 0);
+                         (int64_t)blockIdx.x * 64, 0, tile);
 }
 
 // ---------------------------------------------------------------------------
