@@ -91,7 +91,7 @@ def test_mixed_harness_input_and_reference_entry():
     assert np.array_equal(Q, Q2) and np.array_equal(R, R2)
 
 
-@pytest.mark.parametrize("cond", [1e1, 1e2])
+@pytest.mark.parametrize("cond", [3.0, 1e1, 1e2])
 def test_mixed_conditioned_inputs_stay_on_cholesky_path(cond):
     A = O.gen_conditioned(100000, 32, cond, np.float32, seed=11)
     st, Q, R, _ = call("tsqr_mixed", A)
