@@ -43,6 +43,42 @@ __device__ __forceinline__ f4v load_rows4(const float* __restrict__ A, int64_t l
     }
 }
 
+// FP64 data: rows [r, r + 2) of column c (one 16-byte load), as load_rows4.
+typedef double d2v __attribute__((ext_vector_type(2)));
+template <bool VEC, bool NTL>
+__device__ __forceinline__ d2v load_rows2(const double* __restrict__ A, int64_t lda, int64_t r, int c) {
+    const double* p = A + (int64_t)c * lda + r;
+    if constexpr (VEC && NTL) {
+        return __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+    } else if constexpr (VEC) {
+        return *reinterpret_cast<const d2v*>(p);
+    } else {
+        return d2v{p[0], p[1]};
+    }
+}
+
+// The 16-byte run of rows [r, r + RPL) of column c for either type.
+template <typename T>
+struct Rows16;
+template <>
+struct Rows16<float> {
+    static constexpr int RPL = 4;
+    typedef f4v vec;
+    template <bool VEC, bool NTL>
+    static __device__ __forceinline__ vec load(const float* A, int64_t lda, int64_t r, int c) {
+        return load_rows4<VEC, NTL>(A, lda, r, c);
+    }
+};
+template <>
+struct Rows16<double> {
+    static constexpr int RPL = 2;
+    typedef d2v vec;
+    template <bool VEC, bool NTL>
+    static __device__ __forceinline__ vec load(const double* A, int64_t lda, int64_t r, int c) {
+        return load_rows2<VEC, NTL>(A, lda, r, c);
+    }
+};
+
 // Rows [r, r + 4) of column c, zero past row m.
 __device__ __forceinline__ f4v load_rows4_tail(const float* __restrict__ A, int64_t lda, int64_t r, int c, int m) {
     const float* p = A + (int64_t)c * lda + r;
@@ -53,9 +89,11 @@ __device__ __forceinline__ f4v load_rows4_tail(const float* __restrict__ A, int6
 }
 
 // ---------------------------------------------------------------------------
-// R pass.  Lane l holds rows 4(l/4) .. +3 of the columns 4c + l%4 (c < NC,
-// NC = padded width / 4) of a chunk; step s (0..3) feeds row 4(l/4) + s, so
-// block b of an instruction sums the rows 4(4k + b) + s, k = 0..3.  Every
+// R pass.  Lane l holds rows RPL(l/4) .. +RPL-1 of the columns 4c + l%4
+// (c < NC, NC = padded width / 4) of a chunk of 16 RPL rows (RPL = 4 for
+// FP32 data, 2 for FP64: one 16-byte load per column); step s (0..RPL-1)
+// feeds row RPL(l/4) + s, so block b of an instruction sums the rows
+// RPL(4k + b) + s, k = 0..3.  Every
 // instruction computes the same tile pair (I, J), I <= J, in all four
 // blocks, on different rows; the blocks are added once, at the end
 // (lanes l, l^4, l^8, l^12).  Workgroup b of P (4 waves) takes the chunks
@@ -69,12 +107,17 @@ __device__ __forceinline__ f4v load_rows4_tail(const float* __restrict__ A, int6
 // the memory-side cache, where the Q pass, which starts at the bottom, finds
 // them (measured on MI355X: the nontemporal R pass is 0.09 ms faster over
 // 3.06 GB, the cached bottom saves the Q pass 0.05 ms per 256 MB).
-// FULL: n is the padded width, so no column is padding.
-template <int NC, bool VEC, bool FULL>
-__global__ __launch_bounds__(256) void tsqr_gram_stream4(int m, int n, const float* __restrict__ A, int lda,
+// FULL: n is the padded width, so no column is padding.  FP64 data: the
+// products are rounded FP64 products, as in tsqr_gram_stream<double>.
+template <int NC, bool VEC, bool FULL, typename T = float>
+__global__ __launch_bounds__(256) void tsqr_gram_stream4(int m, int n, const T* __restrict__ A, int lda,
                                                          double* __restrict__ partial, int* __restrict__ tk,
                                                          int cached, const int* __restrict__ skip = nullptr) {
     constexpr int NP = NC * (NC + 1) / 2;
+    using RW = Rows16<T>;
+    using V = typename RW::vec;
+    constexpr int RPL = RW::RPL;
+    constexpr int CR = 16 * RPL;  // rows per chunk
     __shared__ GramLds L;
     __shared__ int bcast;
     if (skip && skip[0]) return;  // an earlier pass already took the flat path
@@ -84,8 +127,8 @@ __global__ __launch_bounds__(256) void tsqr_gram_stream4(int m, int n, const flo
     const int W = 4 * (int)gridDim.x;
     const int w = 4 * (int)blockIdx.x + wv;
     const int z = lane & 3;
-    const int rl = 4 * (lane >> 2);
-    const int nfull = m / 64;
+    const int rl = RPL * (lane >> 2);
+    const int nfull = m / CR;
     double acc[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) acc[p] = 0.0;
@@ -97,9 +140,9 @@ __global__ __launch_bounds__(256) void tsqr_gram_stream4(int m, int n, const flo
         in[c] = FULL || 4 * c + z < n;
         col[c] = in[c] ? 4 * c + z : 0;
     }
-    auto chunk = [&](const f4v(&x)[NC]) {
+    auto chunk = [&](const V(&x)[NC]) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
+        for (int s = 0; s < RPL; ++s) {
             double d[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) d[c] = (FULL || in[c]) ? (double)x[c][s] : 0.0;
@@ -115,12 +158,12 @@ __global__ __launch_bounds__(256) void tsqr_gram_stream4(int m, int n, const flo
     // around loads)
     auto sweep = [&](auto ntl, int k0, int k1) {
         constexpr bool NTL = decltype(ntl)::value;
-        auto load_chunk = [&](int k, f4v(&x)[NC]) {
-            const int64_t r = (int64_t)(w + (k < k1 ? k : k1 - 1) * W) * 64 + rl;
+        auto load_chunk = [&](int k, V(&x)[NC]) {
+            const int64_t r = (int64_t)(w + (k < k1 ? k : k1 - 1) * W) * CR + rl;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) x[c] = load_rows4<VEC, NTL>(A, lda, r, col[c]);
+            for (int c = 0; c < NC; ++c) x[c] = RW::template load<VEC, NTL>(A, lda, r, col[c]);
         };
-        f4v a[NC], b[NC], c[NC];
+        V a[NC], b[NC], c[NC];
         load_chunk(k0, a);
         load_chunk(k0 + 1, b);
 #pragma unroll 1
@@ -149,10 +192,14 @@ __global__ __launch_bounds__(256) void tsqr_gram_stream4(int m, int n, const flo
         if (kc > 0) sweep(std::true_type{}, 0, kc);
         if (kc < nch) sweep(std::false_type{}, kc, nch);
     }
-    if (m % 64 && w == nfull % W) {  // the short last chunk
-        f4v x[NC];
+    if (m % CR && w == nfull % W) {  // the short last chunk
+        V x[NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) x[c] = load_rows4_tail(A, lda, (int64_t)nfull * 64 + rl, col[c], m);
+        for (int c = 0; c < NC; ++c) {
+            const T* p = A + (int64_t)col[c] * lda + (int64_t)nfull * CR + rl;
+#pragma unroll
+            for (int s = 0; s < RPL; ++s) x[c][s] = ((int64_t)nfull * CR + rl + s < m) ? p[s] : T(0);
+        }
         chunk(x);
     }
     // add the four blocks of each tile (fixed order)

@@ -147,10 +147,23 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
                                (const double*)(partial + (size_t)g64 * 1024), R, ldr, side, ctl,
                                mixed_gram64_gate(o.gate), rkeep, 0, (const int*)ctl);
         } else {
-            hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
-                               partial, stream_tk);
-            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
-                               (const double*)stream_sum, R, ldr, side, ctl, o.gate, rkeep);
+            // FP64: the R pass on 4 x 4 tiles from kStream4MinRows rows (two
+            // rows per lane and load), as the FP32 flat path's
+            const bool four = use_stream4(d.m);
+            const int rg = four ? stream_groups4(d.m) : sgroups;
+            if (four) {
+                if (d.n == N)
+                    hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, true, T>), dim3(rg), dim3(256), 0, st, d.m, d.n,
+                                       (const T*)A, lda, partial, stream_tk, 0, (const int*)nullptr);
+                else
+                    hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, false, T>), dim3(rg), dim3(256), 0, st, d.m,
+                                       d.n, (const T*)A, lda, partial, stream_tk, 0, (const int*)nullptr);
+            } else {
+                hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A,
+                                   lda, partial, stream_tk);
+            }
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, stream_group_count(rg),
+                               (const double*)(partial + (size_t)rg * 1024), R, ldr, side, ctl, o.gate, rkeep);
             if (o.r_only) {
                 // second CholeskyQR pass without writing Q1 = A R1^{-1}: the
                 // Gram matrix of the product streamed from A, then the check
