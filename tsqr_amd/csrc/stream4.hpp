@@ -88,6 +88,42 @@ __device__ __forceinline__ f4v load_rows4_tail(const float* __restrict__ A, int6
     return o;
 }
 
+// End of a launch that sums a Gram matrix on 4 x 4 tiles (256 threads; acc
+// holds this lane's tile-pair sums, lane 16k + 4b + z for row k of block b):
+// the four blocks of each tile are added (lanes l, l^4, l^8, l^12), the
+// workgroup's waves in wave order in L.G, then stream_finish.
+template <int NC>
+__device__ __forceinline__ void gram4_finish(GramLds& L, double (&acc)[NC * (NC + 1) / 2],
+                                             double* __restrict__ partial, int* __restrict__ tk, int* bcast) {
+    constexpr int NP = NC * (NC + 1) / 2;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wv = t >> 6;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        acc[p] += __shfl_xor(acc[p], 4);
+        acc[p] += __shfl_xor(acc[p], 8);
+    }
+    for (int e = t; e < 32 * 33; e += 256) (&L.G[0][0])[e] = 0.0;
+    __syncthreads();
+    const int i = lane >> 4, j = lane & 3;
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+        if (wv == s && (lane & 12) == 0) {
+            int p = 0;
+#pragma unroll
+            for (int I = 0; I < NC; ++I)
+#pragma unroll
+                for (int J = I; J < NC; ++J, ++p) {
+                    L.G[4 * I + i][4 * J + j] += acc[p];
+                    if (I != J) L.G[4 * J + j][4 * I + i] += acc[p];
+                }
+        }
+        __syncthreads();
+    }
+    stream_finish(L, partial, tk, bcast);
+}
+
 // ---------------------------------------------------------------------------
 // R pass.  Lane l holds rows RPL(l/4) .. +RPL-1 of the columns 4c + l%4
 // (c < NC, NC = padded width / 4) of a chunk of 16 RPL rows (RPL = 4 for
@@ -202,31 +238,7 @@ __global__ __launch_bounds__(256) void tsqr_gram_stream4(int m, int n, const T* 
         }
         chunk(x);
     }
-    // add the four blocks of each tile (fixed order)
-#pragma unroll
-    for (int p = 0; p < NP; ++p) {
-        acc[p] += __shfl_xor(acc[p], 4);
-        acc[p] += __shfl_xor(acc[p], 8);
-    }
-    // the workgroup's sum in L.G, waves added in order
-    for (int e = t; e < 32 * 33; e += 256) (&L.G[0][0])[e] = 0.0;
-    __syncthreads();
-    const int i = lane >> 4, j = lane & 3;
-#pragma unroll 1
-    for (int s = 0; s < 4; ++s) {
-        if (wv == s && (lane & 12) == 0) {
-            int p = 0;
-#pragma unroll
-            for (int I = 0; I < NC; ++I)
-#pragma unroll
-                for (int J = I; J < NC; ++J, ++p) {
-                    L.G[4 * I + i][4 * J + j] += acc[p];
-                    if (I != J) L.G[4 * J + j][4 * I + i] += acc[p];
-                }
-        }
-        __syncthreads();
-    }
-    stream_finish(L, partial, tk, &bcast);
+    gram4_finish<NC>(L, acc, partial, tk, &bcast);
 }
 
 // ---------------------------------------------------------------------------
@@ -347,6 +359,153 @@ __global__ __launch_bounds__(256) void tsqr_flat_q4(int m, int n, float* __restr
         store_chunk((int64_t)chunk_of(k + 1) * 64, acc);
         __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Q pass of the CholeskyQR2 path on 4 x 4 tiles (FP64 data): Q = A X in
+// place (X = R^{-1} of pass 1 or R2^{-1} of pass 2, FP64 row-major 32 x 32,
+// upper triangular) with the layout of tsqr_flat_q4 for RPL rows per lane
+// (lane l holds rows RPL(l%16) .. +RPL-1 of the columns 4I + l/16; the result
+// of column group J, step s, is Q[RPL(4((l/4)%4) + l/16) + s][4J + l%4]),
+// and in the same pass the Gram matrix of the stored Q on the same
+// instruction: the result layout is the R pass's operand layout (lane
+// 16k + 4b + z holds column 4J + z of one row per step), so step s adds
+// q[I] q[J] to the tile pair (I, J), I <= J, and the partial sums end as
+// tsqr_gram_stream4's (gram4_finish).  A persistent grid (wave w takes the
+// chunks w, w + 4P, ...) with two chunks of loads in flight.  Runs when the
+// flat path was taken (ctl[0]); pass 2 only if the check asked for it
+// (ctl[1]).
+template <int NC, bool VEC, typename T>
+__global__ __launch_bounds__(256) void tsqr_q4_gram(int m, int n, T* __restrict__ A, int lda,
+                                                    const double* __restrict__ X, double* __restrict__ partial,
+                                                    int* __restrict__ tk, const int* __restrict__ ctl, int pass) {
+    constexpr int NP = NC * (NC + 1) / 2;
+    using RW = Rows16<T>;
+    using V = typename RW::vec;
+    constexpr int RPL = RW::RPL;
+    constexpr int CR = 16 * RPL;
+    __shared__ GramLds L;
+    __shared__ int bcast;
+    if (!ctl[0] || (pass == 2 && !ctl[1])) return;
+    const int lane = threadIdx.x & 63;
+    const int W = 4 * (int)gridDim.x;
+    const int w = 4 * (int)blockIdx.x + (threadIdx.x >> 6);
+    const int kc = lane >> 4;
+    const int rl = RPL * (lane & 15);
+    const int jc = lane & 3;
+    const int ro = RPL * (4 * ((lane >> 2) & 3) + (lane >> 4));  // first output row of this lane
+    double mb[NP];
+    {
+        int p = 0;
+#pragma unroll
+        for (int I = 0; I < NC; ++I)
+#pragma unroll
+            for (int J = I; J < NC; ++J, ++p) {
+                const int r = 4 * I + kc, c = 4 * J + jc;
+                mb[p] = (r < n && c < n) ? X[r * 32 + c] : 0.0;
+            }
+    }
+    bool in[NC], out[NC];
+    int col[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        in[c] = 4 * c + kc < n;
+        col[c] = in[c] ? 4 * c + kc : 0;
+        out[c] = 4 * c + jc < n;
+    }
+    double g[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) g[p] = 0.0;
+    // q[s][J]: row ro + s, column 4J + jc, as stored (zero in padded columns
+    // and rows past m)
+    auto chunk = [&](const V(&x)[NC], T (&q)[RPL][NC]) {
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+            double d[NC], a[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                d[c] = in[c] ? (double)x[c][s] : 0.0;
+                a[c] = 0.0;
+            }
+            int p = 0;
+#pragma unroll
+            for (int I = 0; I < NC; ++I)
+#pragma unroll
+                for (int J = I; J < NC; ++J, ++p) a[J] = mfma4x4(d[I], mb[p], a[J]);
+#pragma unroll
+            for (int J = 0; J < NC; ++J) q[s][J] = (T)a[J];
+            p = 0;
+#pragma unroll
+            for (int I = 0; I < NC; ++I)
+#pragma unroll
+                for (int J = I; J < NC; ++J, ++p) g[p] = mfma4x4((double)q[s][I], (double)q[s][J], g[p]);
+        }
+    };
+    auto store_chunk = [&](int64_t r0, const T (&q)[RPL][NC]) {
+#pragma unroll
+        for (int J = 0; J < NC; ++J) {
+            if (!out[J]) continue;
+            T* p = A + (int64_t)(4 * J + jc) * lda + r0 + ro;
+            if constexpr (VEC) {
+                V v;
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) v[s] = q[s][J];
+                *reinterpret_cast<V*>(p) = v;
+            } else {
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) p[s] = q[s][J];
+            }
+        }
+    };
+    const int nfull = m / CR;
+    if (m % CR && w == nfull % W) {  // the short last chunk, bounds-checked
+        V x[NC];
+        const int64_t r0 = (int64_t)nfull * CR;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const T* p = A + (int64_t)col[c] * lda + r0 + rl;
+#pragma unroll
+            for (int s = 0; s < RPL; ++s) x[c][s] = (r0 + rl + s < m) ? p[s] : T(0);
+        }
+        T q[RPL][NC];
+        chunk(x, q);
+#pragma unroll
+        for (int J = 0; J < NC; ++J) {
+            if (!out[J]) continue;
+            T* p = A + (int64_t)(4 * J + jc) * lda + r0 + ro;
+#pragma unroll
+            for (int s = 0; s < RPL; ++s)
+                if (r0 + ro + s < m) p[s] = q[s][J];
+        }
+    }
+    if (w < nfull) {
+        const int nch = (nfull - w + W - 1) / W;
+        auto load_chunk = [&](int k, V(&x)[NC]) {
+            const int64_t r = (int64_t)(w + (k < nch ? k : nch - 1) * W) * CR + rl;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) x[c] = RW::template load<VEC, true>(A, lda, r, col[c]);
+        };
+        // ping-pong buffers; past the last chunk a step reloads the last one
+        // into the buffer it will not use (no branch around loads)
+        V a[NC], b[NC];
+        load_chunk(0, a);
+#pragma unroll 1
+        for (int k = 0; k < nch; k += 2) {
+            T q[RPL][NC];
+            load_chunk(k + 1, b);
+            __builtin_amdgcn_sched_barrier(0);
+            chunk(a, q);
+            store_chunk((int64_t)(w + k * W) * CR, q);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 1 >= nch) break;
+            load_chunk(k + 2, a);
+            __builtin_amdgcn_sched_barrier(0);
+            chunk(b, q);
+            store_chunk((int64_t)(w + (k + 1) * W) * CR, q);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    gram4_finish<NC>(L, g, partial, tk, &bcast);
 }
 
 }  // namespace tsqr

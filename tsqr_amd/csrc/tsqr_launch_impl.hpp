@@ -277,9 +277,19 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
         // fit the workspace)
         const int qgroups = stream_groups4(d.m);
         for (int pass = 1; pass <= 2; ++pass) {
-            hipLaunchKernelGGL((tsqr_mixed_qs<T, NT, VEC>), dim3(qgroups), dim3(256), 0, st, d.m, d.n, A, lda, X0,
-                               partial, ctl + (pass == 1 ? kCtlStreamTickets3 : kCtlStreamTickets4), (const int*)ctl,
-                               pass);
+            int* qtk = ctl + (pass == 1 ? kCtlStreamTickets3 : kCtlStreamTickets4);
+            bool q4 = false;
+            if constexpr (sizeof(T) == 8) {
+                if (use_stream4(d.m)) {
+                    // FP64 data: products and the Gram of Q on 4 x 4 tiles
+                    hipLaunchKernelGGL((tsqr_q4_gram<N / 4, VEC, T>), dim3(qgroups), dim3(256), 0, st, d.m, d.n, A,
+                                       lda, X0, partial, qtk, (const int*)ctl, pass);
+                    q4 = true;
+                }
+            }
+            if (!q4)
+                hipLaunchKernelGGL((tsqr_mixed_qs<T, NT, VEC>), dim3(qgroups), dim3(256), 0, st, d.m, d.n, A, lda,
+                                   X0, partial, qtk, (const int*)ctl, pass);
             hipLaunchKernelGGL(tsqr_mixed_check<T>, dim3(1), dim3(kSumThreads), 0, st, d, stream_group_count(qgroups),
                                (const double*)(partial + (size_t)qgroups * 1024), R, ldr, side, ctl, pass, o.tol);
         }
