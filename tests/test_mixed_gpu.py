@@ -119,6 +119,16 @@ def test_mixed_ill_conditioned_escalates_to_fp64_gram(cond):
     check_mixed(A, Q, R, f"mixed cond={cond:g} FP64 Gram")
 
 
+def test_mixed_escalation_tall_padded_width():
+    """The escalation's FP64 Gram on 4 x 4 tiles (600000 rows) with padding
+    columns in the last column group (n = 20)."""
+    A = O.gen_conditioned(600000, 20, 1e6, np.float32, seed=18)
+    st, Q, R, _ = call("tsqr_mixed", A)
+    assert st == 0
+    assert LAST_MIXED_STATS["flat"] == 1 and LAST_MIXED_STATS["bf16_q"][0] == 0
+    check_mixed(A, Q, R, "mixed 600000x20 cond=1e6 FP64 Gram")
+
+
 def test_mixed_beyond_fp64_gate_falls_back():
     """Numerically rank deficient in FP32 (cond 1e9): both gates reject,
     the Householder tree runs, with its kappa-independent bounds."""
