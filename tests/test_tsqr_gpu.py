@@ -88,18 +88,19 @@ def test_shapes(m, n, dtype, path):
 
 @pytest.mark.parametrize("path", [pytest.param(AUTO, id="auto"), pytest.param(CHOLQR2, id="cholqr2")])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("n", [5, 13, 20])
-def test_tall_padded_widths(n, dtype, path):
+@pytest.mark.parametrize("m,n", [(600000, 5), (600000, 13), (600000, 20), (524301, 32), (524301, 20)])
+def test_tall_padded_widths(m, n, dtype, path):
     """From 524,288 rows the flat paths run on 4 x 4 FP64 tiles (the R pass,
     and the Q pass of FP32 AUTO and of FP64 CholeskyQR2): padded widths 8
-    and 32, and padding columns inside the last column group."""
-    A = O.gen_normal(600000, n, dtype, seed=n * 11 + 1)
+    and 32, padding columns inside the last column group, and a short last
+    chunk (524301 = 64 * 8192 + 13 rows)."""
+    A = O.gen_normal(m, n, dtype, seed=n * 11 + m % 7)
     with LeafPath(dtype, path):
         st, Q, R, _ = call("tsqr", A)
     assert st == 0
     assert LAST_LEAF_STATS["flat"] == 1
     floor = 32 if dtype == np.float64 else 1
-    check(A, Q, R, f"600000x{n} {np.dtype(dtype).name} path={path} leaves={LAST_LEAF_STATS}", n_floor=floor)
+    check(A, Q, R, f"{m}x{n} {np.dtype(dtype).name} path={path} leaves={LAST_LEAF_STATS}", n_floor=floor)
 
 
 @pytest.mark.parametrize("path", [pytest.param(AUTO, id="auto"), pytest.param(CHOLQR2, id="cholqr2")])
