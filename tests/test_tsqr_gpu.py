@@ -105,6 +105,21 @@ def test_tall_padded_widths(m, n, dtype, path):
 
 @pytest.mark.parametrize("path", [pytest.param(AUTO, id="auto"), pytest.param(CHOLQR2, id="cholqr2")])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_tall_unaligned_lda(dtype, path):
+    """The 4 x 4-tile passes without 16-byte loads: lda = m + 37 (the
+    columns start at every alignment), short last chunk, padded width."""
+    m, n = 524301, 20
+    A = O.gen_normal(m, n, dtype, seed=23)
+    with LeafPath(dtype, path):
+        st, Q, R, pad_ok = call("tsqr", A, lda=m + 37)
+    assert st == 0 and pad_ok
+    assert LAST_LEAF_STATS["flat"] == 1
+    floor = 32 if dtype == np.float64 else 1
+    check(A, Q, R, f"{m}x{n} lda+37 {np.dtype(dtype).name} path={path}", n_floor=floor)
+
+
+@pytest.mark.parametrize("path", [pytest.param(AUTO, id="auto"), pytest.param(CHOLQR2, id="cholqr2")])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("m,n", SHAPES)
 def test_flat_paths_forced_to_fall_back(m, n, dtype, path):
     """The smallest gate (1; ||R||_F ||R^-1||_F >= sqrt(n) > 1 for n >= 2)
