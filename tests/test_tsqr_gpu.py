@@ -119,6 +119,22 @@ def test_tall_unaligned_lda(dtype, path):
 
 
 @pytest.mark.parametrize("path", [pytest.param(AUTO, id="auto"), pytest.param(CHOLQR2, id="cholqr2")])
+@pytest.mark.parametrize("m,n,dtype", [
+    (65536, 32, np.float32), (65600, 32, np.float32), (65536, 13, np.float32),
+    (32768, 32, np.float64), (32800, 32, np.float64), (32768, 20, np.float64)])
+def test_short_r_pass_boundary(m, n, dtype, path):
+    """Either side of the ticket-free R pass on 16-wave workgroups (64
+    workgroups of 16 chunks: 65536 FP32 rows, 32768 FP64 rows): the short
+    form and the ticketed streaming form both meet the bounds."""
+    A = O.gen_uniform(m, n, dtype, seed=m + n)
+    with LeafPath(dtype, path):
+        st, Q, R, _ = call("tsqr", A)
+    assert st == 0 and LAST_LEAF_STATS["flat"] == 1
+    floor = 32 if dtype == np.float64 else 1
+    check(A, Q, R, f"{m}x{n} {np.dtype(dtype).name} path={path}", n_floor=floor)
+
+
+@pytest.mark.parametrize("path", [pytest.param(AUTO, id="auto"), pytest.param(CHOLQR2, id="cholqr2")])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("m,n", SHAPES)
 def test_flat_paths_forced_to_fall_back(m, n, dtype, path):

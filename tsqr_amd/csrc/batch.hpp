@@ -140,50 +140,6 @@ constexpr int batch_super_threads() {
     return 64 * (a > b ? (a > c ? a : c) : (b > c ? b : c));
 }
 
-// L.G = the sum of `count` 32 x 32 FP64 blocks at src (block b at b * stride
-// blocks), by a workgroup of four waves (256 threads): wave w adds the
-// blocks [w c / 4, (w + 1) c / 4) in order, four blocks in flight (they were
-// written by other CUs, so every load goes to memory), then the four wave
-// sums are added in wave order in LDS, so the result is fixed for a given
-// count.  Every thread of the workgroup must call it.
-__device__ __forceinline__ void sum_blocks_wg4(GramLds& L, const double* __restrict__ src, int count,
-                                               int stride = 1) {
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int b0 = (int)((int64_t)count * w / 4), b1 = (int)((int64_t)count * (w + 1) / 4);
-    double s[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) s[e] = 0.0;
-    int b = b0;
-#pragma unroll 1
-    for (; b + 4 <= b1; b += 4) {
-        double v[4][16];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int e = 0; e < 16; ++e) v[q][e] = src[(int64_t)(b + q) * stride * 1024 + lane + 64 * e];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int e = 0; e < 16; ++e) s[e] += v[q][e];
-    }
-#pragma unroll 1
-    for (; b < b1; ++b)
-#pragma unroll
-        for (int e = 0; e < 16; ++e) s[e] += src[(int64_t)b * stride * 1024 + lane + 64 * e];
-#pragma unroll 1
-    for (int k = 0; k < 4; ++k) {
-        if (w == k) {
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int idx = lane + 64 * e;
-                L.G[idx >> 5][idx & 31] = (k == 0 ? 0.0 : L.G[idx >> 5][idx & 31]) + s[e];
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // ctl words of a problem (int): 0 flat CholeskyQR taken, 1 second pass
 // asked, 2 check status, 3 / 4 BF16 Q passes (mixed.hpp).
 

@@ -250,16 +250,19 @@ __global__ __launch_bounds__(256) void gemm_tn_reduce(int n1, int n2, int chunks
     if (sym && out < total && (out % n1) / 64 > (out / n1) / 64) idx = (out / n1) + (out % n1) * n1;
     TS s = TS(0);
     if (idx < total) {
-        int k = g;
+        // eight chunks in flight per step, the last step's too (its loads
+        // past the end reread chunk g and add zero)
 #pragma unroll 1
-        for (; k + 7 * 16 < chunks; k += 8 * 16) {
+        for (int k = g; k < chunks; k += 8 * 16) {
             T v[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = part[(int64_t)(k + 16 * q) * total + idx];
+            for (int q = 0; q < 8; ++q) {
+                const int kq = k + 16 * q < chunks ? k + 16 * q : g;
+                v[q] = part[(int64_t)kq * total + idx];
+            }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) s += (TS)v[q];
+            for (int q = 0; q < 8; ++q) s += k + 16 * q < chunks ? (TS)v[q] : TS(0);
         }
-        for (; k < chunks; k += 16) s += (TS)part[(int64_t)k * total + idx];
     }
     red[o][g] = s;
     __syncthreads();

@@ -513,57 +513,83 @@ __global__ __launch_bounds__(64) void tsqr_gram_q(const TreeDesc d, T* __restric
 
 // Sums `count` 32 x 32 FP64 blocks at src in index order (256 threads):
 // out[e] is entry t + 256 e of the sum.  Eight blocks are in flight per
-// step (the blocks come from other CUs, so every load goes to memory).
+// step (the blocks come from other CUs, so every load goes to memory),
+// the last step's too: its loads past the end reread the last block and
+// add zero, so a short sum is one round trip to memory, not one per block.
 __device__ __forceinline__ void sum_blocks(const double* __restrict__ src, int count, int t, double (&out)[4]) {
     double s[4] = {0, 0, 0, 0};
-    int b = 0;
 #pragma unroll 1
-    for (; b + 8 <= count; b += 8) {
+    for (int b = 0; b < count; b += 8) {
         double v[8][4];
 #pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int bq = b + q < count ? b + q : count - 1;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[q][e] = src[(int64_t)bq * 1024 + t + 256 * e];
+        }
+#pragma unroll
         for (int q = 0; q < 8; ++q)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) v[q][e] = src[(int64_t)(b + q) * 1024 + t + 256 * e];
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) s[e] += v[q][e];
+            for (int e = 0; e < 4; ++e) s[e] += b + q < count ? v[q][e] : 0.0;
     }
-    for (; b < count; ++b)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) s[e] += src[(int64_t)b * 1024 + t + 256 * e];
 #pragma unroll
     for (int e = 0; e < 4; ++e) out[e] = s[e];
 }
 
 // L.G = sum of the nparts 32 x 32 FP64 partials, added in index order
-// (kSumThreads = 256 threads, thread t owns entries t + 256 e).  The
-// partials were written by other CUs, so every load goes to memory: eight
-// partials (32 loads per thread) are in flight at a time.  The caller
-// synchronises before reading L.G.
+// (kSumThreads = 256 threads, thread t owns entries t + 256 e; sum_blocks).
+// The caller synchronises before reading L.G.
 constexpr int kSumThreads = 256;
 __device__ __forceinline__ void sum_partials(GramLds& L, int nparts, const double* __restrict__ partial, int t) {
-    double s[4] = {0, 0, 0, 0};
-    int p = 0;
-#pragma unroll 1
-    for (; p + 8 <= nparts; p += 8) {
-        double v[8][4];
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) v[q][e] = partial[(int64_t)(p + q) * 1024 + t + 256 * e];
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) s[e] += v[q][e];
-    }
-    for (; p < nparts; ++p)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) s[e] += partial[(int64_t)p * 1024 + t + 256 * e];
+    double s[4];
+    sum_blocks(partial, nparts, t, s);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int idx = t + 256 * e;
         L.G[idx >> 5][idx & 31] = s[e];
+    }
+}
+
+// L.G = the sum of `count` 32 x 32 FP64 blocks at src (block b at b * stride
+// blocks), by a workgroup of four waves (256 threads): wave w adds the
+// blocks [w c / 4, (w + 1) c / 4) in order, four blocks in flight (they were
+// written by other CUs, so every load goes to memory), then the four wave
+// sums are added in wave order in LDS, so the result is fixed for a given
+// count.  Every thread of the workgroup must call it.
+__device__ __forceinline__ void sum_blocks_wg4(GramLds& L, const double* __restrict__ src, int count,
+                                               int stride = 1) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int b0 = (int)((int64_t)count * w / 4), b1 = (int)((int64_t)count * (w + 1) / 4);
+    double s[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) s[e] = 0.0;
+    // four blocks in flight per step, the last step's too (its loads past
+    // b1 reread block b1 - 1 and add zero)
+#pragma unroll 1
+    for (int b = b0; b < b1; b += 4) {
+        double v[4][16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int bq = b + q < b1 ? b + q : b1 - 1;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[q][e] = src[(int64_t)bq * stride * 1024 + lane + 64 * e];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) s[e] += b + q < b1 ? v[q][e] : 0.0;
+    }
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        if (w == k) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int idx = lane + 64 * e;
+                L.G[idx >> 5][idx & 31] = (k == 0 ? 0.0 : L.G[idx >> 5][idx & 31]) + s[e];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -573,31 +599,42 @@ __device__ __forceinline__ void sum_partials(GramLds& L, int nparts, const doubl
 // diagonal); lane 32 + c starts from column c of the identity and, by the
 // same row operations (forward substitution with R^T), ends with column c
 // of R^{-T}, i.e. row c of R^{-1}.  Step j scales row j by 1/R_jj and
-// subtracts R_jr times it from every row r > j; R_jr is broadcast through
-// L.row.  The whole symmetric Schur complement is updated (no per-lane
-// select); its part below the diagonal is zeroed as each step reaches it.
-// Returns false (in every lane) if a pivot is not positive and finite; the
-// factor is then meaningless.
+// subtracts R_jr times it from every row r > j.  The step's critical path
+// is short: row j + 1 is updated first, with R_j,j+1 read from lane j + 1,
+// and the next pivot's 1/sqrt is formed right away; the other R_jr reach
+// every lane through L.row (LDS instructions of a wave execute in order, so
+// the next step's write cannot pass this step's reads).  On MI355X: 12.7k
+// cycles against 15.7k with the whole row broadcast before each pivot, bit
+// for bit the same results.  The whole symmetric Schur complement is
+// updated (no per-lane select); its part below the diagonal is zeroed as
+// each step reaches it.  Returns false (in every lane) if a pivot is not
+// positive and finite; the factor is then meaningless.
 __device__ __forceinline__ bool chol_inv_wave(GramLds& L, int n, int lane, double (&g)[32]) {
     const int c = lane & 31;
     const bool hi = lane >= 32;
 #pragma unroll
     for (int r = 0; r < 32; ++r) g[r] = (!hi && r < n && c < n) ? L.G[r][c] : (r == c ? 1.0 : 0.0);
     bool ok = true;
-#pragma unroll
-    for (int j = 0; j < 32; ++j) {
+    auto pivot = [&](int j) {
         double piv = readlane_t<double>(g[j], j);
         if (!(piv > 0.0) || !(piv < __builtin_inf())) {
             ok = false;
             piv = 1.0;
         }
-        const double rd = rsqrt_fp64(piv);
+        return rsqrt_fp64(piv);
+    };
+    double rd = pivot(0);
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
         g[j] = (hi || c >= j) ? g[j] * rd : 0.0;
         if (!hi) L.row[c] = g[j];
+        if (j + 1 < 32) {
+            g[j + 1] = fmat(-readlane_t<double>(g[j], j + 1), g[j], g[j + 1]);
+            rd = pivot(j + 1);
+        }
         wave_sync();
 #pragma unroll
-        for (int r = j + 1; r < 32; ++r) g[r] = fmat(-L.row[r], g[j], g[r]);
-        wave_sync();
+        for (int r = j + 2; r < 32; ++r) g[r] = fmat(-L.row[r], g[j], g[r]);
     }
     return ok;
 }
@@ -761,8 +798,9 @@ __device__ __forceinline__ bool last_group(int* tk, int count, int* bcast) {
 // after the workgroup's sum is in L.G (synchronised): L.G goes to
 // partial[blockIdx.x]; the last workgroup of each group of G =
 // stream_group_size(P) (ticket tk[g]) sums the group's partials in index
-// order into partial[P + g].
-__device__ __forceinline__ void stream_finish(GramLds& L, double* __restrict__ partial, int* __restrict__ tk,
+// order into partial[P + g].  Returns (in every thread) whether this
+// workgroup wrote a group sum.
+__device__ __forceinline__ bool stream_finish(GramLds& L, double* __restrict__ partial, int* __restrict__ tk,
                                               int* bcast) {
     const int t = threadIdx.x;
     const int P = gridDim.x;
@@ -776,22 +814,23 @@ __device__ __forceinline__ void stream_finish(GramLds& L, double* __restrict__ p
     const int g = blockIdx.x / G;
     const int g0 = g * G;
     const int members = P - g0 < G ? P - g0 : G;
-    if (!last_group(tk + g, members, bcast)) return;
+    if (!last_group(tk + g, members, bcast)) return false;
     double sum[4];
     sum_blocks(partial + (int64_t)g0 * 1024, members, t, sum);
     double* gsum = partial + (int64_t)(P + g) * 1024;
 #pragma unroll
     for (int e = 0; e < 4; ++e) gsum[t + 256 * e] = sum[e];
+    return true;
 }
 
 // End of a streaming Gram launch (256 threads, 4 waves, P = gridDim.x
 // workgroups): each wave holds its FP64 MFMA accumulators of the blocks
 // (0,0), (0,1), (1,1) of a 32 x 32 Gram matrix (lane l, register v: row
 // 4v + l/16, column l%16 of the block).  The workgroup sums its waves in
-// wave order into partial[blockIdx.x]; then stream_finish.
+// wave order in L.G (stream_lds_sum), then stream_reduce writes it to
+// partial[blockIdx.x] and runs stream_finish, whose result it returns.
 template <int NT>
-__device__ __forceinline__ void stream_reduce(GramLds& L, const v4d& a00, const v4d& a01, const v4d& a11,
-                                              double* __restrict__ partial, int* __restrict__ tk, int* bcast) {
+__device__ __forceinline__ void stream_lds_sum(GramLds& L, const v4d& a00, const v4d& a01, const v4d& a11) {
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wv = t >> 6;
@@ -820,31 +859,27 @@ __device__ __forceinline__ void stream_reduce(GramLds& L, const v4d& a00, const 
         }
         __syncthreads();
     }
-    stream_finish(L, partial, tk, bcast);
 }
 
-// partial: P + stream_group_count(P) blocks of 32 x 32 FP64; tk: the
-// kCtlStreamTickets ticket words, zero on entry.
+template <int NT>
+__device__ __forceinline__ bool stream_reduce(GramLds& L, const v4d& a00, const v4d& a01, const v4d& a11,
+                                              double* __restrict__ partial, int* __restrict__ tk, int* bcast) {
+    stream_lds_sum<NT>(L, a00, a01, a11);
+    return stream_finish(L, partial, tk, bcast);
+}
+
+// The sweep of a streaming Gram launch: wave w of W sums the products of
+// its chunks (w, w + W, ...) into its FP64 MFMA accumulators a00, a01, a11.
 template <typename T, int NT, bool VEC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void tsqr_gram_stream(const TreeDesc d, const T* __restrict__ A, int lda,
-                                                         double* __restrict__ partial, int* __restrict__ tk,
-                                                         const int* __restrict__ skip = nullptr) {
-    __shared__ GramLds L;
-    __shared__ int bcast;
-    if (skip && skip[0]) return;  // an earlier pass already took the flat path
-    const int t = threadIdx.x;
-    const int lane = t & 63;
-    const int wv = t >> 6;
+__device__ __forceinline__ void gram_stream_acc(const TreeDesc& d, const T* __restrict__ A, int lda, int W, int w,
+                                                v4d& a00, v4d& a01, v4d& a11) {
+    const int lane = threadIdx.x & 63;
     const int m = d.m, n = d.n;
-    const int P = gridDim.x;
-    const int W = 4 * P;
-    const int w = 4 * blockIdx.x + wv;
     const int col = lane & 15;
     const int rq = 4 * (lane >> 4);
     constexpr int S = stream_slabs<T>();
     constexpr int CR = 16 * S;  // rows per chunk
     const int nfull = m / CR;   // complete chunks
-    v4d a00 = {0, 0, 0, 0}, a01 = {0, 0, 0, 0}, a11 = {0, 0, 0, 0};
     // padding columns (n < 32) load column 0 and are dropped in gram_chunk
     const bool in0 = col < n, in1 = col + 16 < n;
     const int c0col = in0 ? col : 0, c1col = in1 ? col + 16 : 0;
@@ -891,7 +926,157 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
         }
         gram_chunk<T, NT, S>(c0, c1, a00, a01, a11);
     }
+}
+
+// partial: P + stream_group_count(P) blocks of 32 x 32 FP64; tk: the
+// kCtlStreamTickets ticket words, zero on entry.
+template <typename T, int NT, bool VEC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void tsqr_gram_stream(const TreeDesc d, const T* __restrict__ A, int lda,
+                                                         double* __restrict__ partial, int* __restrict__ tk,
+                                                         const int* __restrict__ skip = nullptr) {
+    __shared__ GramLds L;
+    __shared__ int bcast;
+    if (skip && skip[0]) return;  // an earlier pass already took the flat path
+    v4d a00 = {0, 0, 0, 0}, a01 = {0, 0, 0, 0}, a11 = {0, 0, 0, 0};
+    gram_stream_acc<T, NT, VEC>(d, A, lda, 4 * (int)gridDim.x, 4 * (int)blockIdx.x + (int)(threadIdx.x >> 6), a00,
+                                a01, a11);
     stream_reduce<NT>(L, a00, a01, a11, partial, tk, &bcast);
+}
+
+// ---------------------------------------------------------------------------
+// Short matrices (plan.hpp, use_stream_small): the R pass on workgroups of
+// 16 waves with one chunk per wave (tsqr_gram_stream16) and no tickets;
+// each workgroup sums its waves in LDS and writes partial[blockIdx.x], and
+// tsqr_gram_flat16 sums the P partials with 8 waves (up to four blocks
+// each, in flight together) and factors.  Measured on MI355X for 27648 x 32: the ticketed
+// 4-wave launch spends about 8 of its 12-15 us after its products (the
+// release fence before each ticket, about 3.5 us, the acquire and the group
+// sums); 4-wave workgroups without tickets leave four times the partials,
+// which one workgroup cannot sum in one round trip.
+
+// Sums the 16 waves' FP64 MFMA accumulators (stream_lds_sum's layout) in
+// four LDS blocks (wave w adds into Q[w % 4], in the order w / 4), then
+// returns entry t = threadIdx.x of the total, ((Q0 + Q1) + Q2) + Q3.
+template <int NT>
+__device__ __forceinline__ double stream_lds_sum16(GramLds (&Q)[4], const v4d& a00, const v4d& a01, const v4d& a11) {
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wv = t >> 6;
+    const int col = lane & 15;
+    const int i0 = lane >> 4;
+    GramLds& L = Q[wv & 3];
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+        if ((wv >> 2) == s) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 4 * v + i0;
+                const double g01 = NT == 2 ? a01[v] : 0.0;
+                const double g11 = NT == 2 ? a11[v] : 0.0;
+                if (s == 0) {
+                    L.G[i][col] = a00[v];
+                    L.G[i][16 + col] = g01;
+                    L.G[16 + col][i] = g01;
+                    L.G[16 + i][16 + col] = g11;
+                } else {
+                    L.G[i][col] += a00[v];
+                    L.G[i][16 + col] += g01;
+                    L.G[16 + col][i] += g01;
+                    L.G[16 + i][16 + col] += g11;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const int r = t >> 5, c = t & 31;
+    return ((Q[0].G[r][c] + Q[1].G[r][c]) + Q[2].G[r][c]) + Q[3].G[r][c];
+}
+
+// partial: gridDim.x blocks of 32 x 32 FP64.
+template <typename T, int NT, bool VEC>
+__global__ __launch_bounds__(1024) void tsqr_gram_stream16(const TreeDesc d, const T* __restrict__ A, int lda,
+                                                           double* __restrict__ partial,
+                                                           const int* __restrict__ skip = nullptr) {
+    __shared__ GramLds Q[4];
+    if (skip && skip[0]) return;  // an earlier pass already took the flat path
+    v4d a00 = {0, 0, 0, 0}, a01 = {0, 0, 0, 0}, a11 = {0, 0, 0, 0};
+    gram_stream_acc<T, NT, VEC>(d, A, lda, 16 * (int)gridDim.x, 16 * (int)blockIdx.x + (int)(threadIdx.x >> 6), a00,
+                                a01, a11);
+    partial[(int64_t)blockIdx.x * 1024 + threadIdx.x] = stream_lds_sum16<NT>(Q, a00, a01, a11);
+}
+
+// Q[0].G = the sum of `count` 32 x 32 FP64 blocks at src by 8 waves (512
+// threads): wave w adds the blocks [w c / 8, (w + 1) c / 8) in order, four
+// in flight per step, then the wave sums are added in four LDS blocks (wave
+// w into Q[w % 4], in the order w / 4) and those as ((Q0 + Q1) + Q2) + Q3.
+// Every thread of the workgroup must call it; Q[0] is synchronised on return.
+constexpr int kFlat16Threads = 512;
+__device__ __forceinline__ void sum_blocks_wg8(GramLds (&Q)[4], const double* __restrict__ src, int count) {
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int w = t >> 6;
+    const int b0 = count * w / 8, b1 = count * (w + 1) / 8;
+    double s[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) s[e] = 0.0;
+    // four blocks in flight per step, the last step's too (its loads past b1
+    // reread block b1 - 1 and add zero)
+#pragma unroll 1
+    for (int b = b0; b < b1; b += 4) {
+        double v[4][16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int bq = b + q < b1 ? b + q : b1 - 1;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[q][e] = src[(int64_t)bq * 1024 + lane + 64 * e];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) s[e] += b + q < b1 ? v[q][e] : 0.0;
+    }
+    GramLds& L = Q[w & 3];
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        if ((w >> 2) == k) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int idx = lane + 64 * e;
+                L.G[idx >> 5][idx & 31] = (k == 0 ? 0.0 : L.G[idx >> 5][idx & 31]) + s[e];
+            }
+        }
+        __syncthreads();
+    }
+    double sum[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int idx = t + kFlat16Threads * h;
+        const int r = idx >> 5, c = idx & 31;
+        sum[h] = ((Q[0].G[r][c] + Q[1].G[r][c]) + Q[2].G[r][c]) + Q[3].G[r][c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int idx = t + kFlat16Threads * h;
+        Q[0].G[idx >> 5][idx & 31] = sum[h];
+    }
+    __syncthreads();
+}
+
+// Flat factor of a short matrix (one workgroup of 8 waves): G = the sum of
+// the nparts partials of tsqr_gram_stream16, then flat_factor as
+// tsqr_gram_flat.
+template <typename T>
+__global__ __launch_bounds__(kFlat16Threads) void tsqr_gram_flat16(const TreeDesc d, int nparts,
+                                                         const double* __restrict__ partial, T* __restrict__ R,
+                                                         int ldr, double* __restrict__ side, int* __restrict__ ctl,
+                                                         double gate, double* __restrict__ rkeep,
+                                                         const int* __restrict__ skip = nullptr) {
+    __shared__ GramLds Q[4];
+    if (skip && skip[0]) return;  // an earlier pass already took the flat path
+    sum_blocks_wg8(Q, partial, nparts);
+    if (threadIdx.x >= 64) return;
+    flat_factor<T>(Q[0], d.n, R, ldr, side + (int64_t)d.side_leaves * 1024, ctl, gate, rkeep, threadIdx.x);
 }
 
 }  // namespace tsqr

@@ -176,6 +176,22 @@ inline int stream_groups(int64_t m) {
     return (int)(p < cap ? p : cap);
 }
 
+// Short matrices: the R pass without tickets on workgroups of 16 waves, one
+// chunk of chunk_rows rows per wave (tsqr_gram_stream16), at most
+// kStreamSmallGroups workgroups, whose partials tsqr_gram_flat16 sums with
+// 16 waves.  A ticketed launch spends about 8 us of a 27648 x 32 call on the
+// release fence, the ticket and the group sums.  The workspace holds the
+// partials: stream_groups(m) >= stream_small_groups(m, .) here.
+constexpr int kStreamSmallGroups = 64;
+inline int stream_small_groups(int64_t m, int chunk_rows) {
+    const int64_t chunks = (m + chunk_rows - 1) / chunk_rows;
+    const int64_t p = (chunks + 15) / 16;
+    return (int)(p < 1 ? 1 : (p > kStreamSmallGroups + 1 ? kStreamSmallGroups + 1 : p));
+}
+inline bool use_stream_small(int64_t m, int chunk_rows) {
+    return stream_small_groups(m, chunk_rows) <= kStreamSmallGroups;
+}
+
 // The FP32 flat path's passes on 4 x 4 tiles (stream4.hpp).  The R pass
 // (tsqr_gram_stream4) takes over from tsqr_gram_stream at kStream4MinRows
 // rows: below that its longer reduction at the end cost more than its

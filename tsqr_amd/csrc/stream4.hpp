@@ -254,14 +254,12 @@ __global__ __launch_bounds__(256) void tsqr_gram_stream4(int m, int n, const T* 
 // column: one 16-byte store.  A persistent grid sweeps the chunks from the
 // bottom of A up: the R pass read them last, so part of them is still in
 // the memory-side cache.
+// flat_q4_body is the sweep of wave w of W; tsqr_flat_q4 the launch.
 template <int NC, bool VEC, bool NTL>
-__global__ __launch_bounds__(256) void tsqr_flat_q4(int m, int n, float* __restrict__ A, int lda,
-                                                    const double* __restrict__ X, const int* __restrict__ ctl) {
+__device__ __forceinline__ void flat_q4_body(int m, int n, float* __restrict__ A, int lda,
+                                             const double* __restrict__ X, int W, int w) {
     constexpr int NP = NC * (NC + 1) / 2;
-    if (!ctl[0]) return;
     const int lane = threadIdx.x & 63;
-    const int W = 4 * (int)gridDim.x;
-    const int w = 4 * (int)blockIdx.x + (threadIdx.x >> 6);
     const int kc = lane >> 4;
     const int rl = 4 * (lane & 15);
     const int jc = lane & 3;
@@ -359,6 +357,13 @@ __global__ __launch_bounds__(256) void tsqr_flat_q4(int m, int n, float* __restr
         store_chunk((int64_t)chunk_of(k + 1) * 64, acc);
         __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+template <int NC, bool VEC, bool NTL>
+__global__ __launch_bounds__(256) void tsqr_flat_q4(int m, int n, float* __restrict__ A, int lda,
+                                                    const double* __restrict__ X, const int* __restrict__ ctl) {
+    if (!ctl[0]) return;
+    flat_q4_body<NC, VEC, NTL>(m, n, A, lda, X, 4 * (int)gridDim.x, 4 * (int)blockIdx.x + (int)(threadIdx.x >> 6));
 }
 
 // ---------------------------------------------------------------------------

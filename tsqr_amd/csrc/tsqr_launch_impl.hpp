@@ -118,6 +118,26 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     const int sgroup_parts = stream_group_count(sgroups);
     double* stream_sum = partial + (size_t)sgroups * 1024;
     const double* X0 = side + (size_t)d.side_leaves * 1024;  // R^{-1} of the flat path
+    // Short matrices (use_stream_small): the R pass without tickets on
+    // ssmall workgroups of 16 waves, summed by the flat factor's 8 waves.
+    constexpr int kChunkRows = 16 * stream_slabs<T>();
+    const bool small = use_stream_small(d.m, kChunkRows);
+    const int ssmall = stream_small_groups(d.m, kChunkRows);
+    // the R pass on 16 x 16 tiles (FP64 products) and the flat factor
+    auto gram_and_factor = [&](int* tk, double gate, double* keep, const int* skip) {
+        if (small) {
+            hipLaunchKernelGGL((tsqr_gram_stream16<T, NT, VEC>), dim3(ssmall), dim3(1024), 0, st, d, (const T*)A,
+                               lda, partial, skip);
+            hipLaunchKernelGGL((tsqr_gram_flat16<T>), dim3(1), dim3(kFlat16Threads), 0, st, d, ssmall,
+                               (const double*)partial,
+                               R, ldr, side, ctl, gate, keep, skip);
+        } else {
+            hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
+                               partial, tk, skip);
+            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, sgroup_parts,
+                               (const double*)stream_sum, R, ldr, side, ctl, gate, keep, 0, skip);
+        }
+    };
     if (mixed) {
         if constexpr (sizeof(T) == 4) {
             hipLaunchKernelGGL((tsqr_mixed_stream<NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const float*)A, lda,
@@ -130,40 +150,36 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
             // first pass succeeded (ctl[0]).  Pass 1 then forms Q1 with FP64
             // products and pass 2 restores the orthogonality.
             int* tk64 = ctl + kCtlStreamTickets2;
-            const bool four = use_stream4(d.m);
-            const int g64 = four ? stream_groups4(d.m) : sgroups;
-            if (four) {
+            if (use_stream4(d.m)) {
+                const int g64 = stream_groups4(d.m);
                 if (d.n == N)
                     hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, true>), dim3(g64), dim3(256), 0, st, d.m, d.n,
                                        (const float*)A, lda, partial, tk64, kCachedChunks, (const int*)ctl);
                 else
                     hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, false>), dim3(g64), dim3(256), 0, st, d.m, d.n,
                                        (const float*)A, lda, partial, tk64, kCachedChunks, (const int*)ctl);
+                hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d,
+                                   stream_group_count(g64), (const double*)(partial + (size_t)g64 * 1024), R, ldr,
+                                   side, ctl, mixed_gram64_gate(o.gate), rkeep, 0, (const int*)ctl);
             } else {
-                hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(g64), dim3(256), 0, st, d, (const T*)A, lda,
-                                   partial, tk64, (const int*)ctl);
+                gram_and_factor(tk64, mixed_gram64_gate(o.gate), rkeep, (const int*)ctl);
             }
-            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, stream_group_count(g64),
-                               (const double*)(partial + (size_t)g64 * 1024), R, ldr, side, ctl,
-                               mixed_gram64_gate(o.gate), rkeep, 0, (const int*)ctl);
         } else {
             // FP64: the R pass on 4 x 4 tiles from kStream4MinRows rows (two
             // rows per lane and load), as the FP32 flat path's
-            const bool four = use_stream4(d.m);
-            const int rg = four ? stream_groups4(d.m) : sgroups;
-            if (four) {
+            if (use_stream4(d.m)) {
+                const int rg = stream_groups4(d.m);
                 if (d.n == N)
                     hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, true, T>), dim3(rg), dim3(256), 0, st, d.m, d.n,
                                        (const T*)A, lda, partial, stream_tk, 0, (const int*)nullptr);
                 else
                     hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, false, T>), dim3(rg), dim3(256), 0, st, d.m,
                                        d.n, (const T*)A, lda, partial, stream_tk, 0, (const int*)nullptr);
+                hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, stream_group_count(rg),
+                                   (const double*)(partial + (size_t)rg * 1024), R, ldr, side, ctl, o.gate, rkeep);
             } else {
-                hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A,
-                                   lda, partial, stream_tk);
+                gram_and_factor(stream_tk, o.gate, rkeep, (const int*)nullptr);
             }
-            hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, stream_group_count(rg),
-                               (const double*)(partial + (size_t)rg * 1024), R, ldr, side, ctl, o.gate, rkeep);
             if (o.r_only) {
                 // second CholeskyQR pass without writing Q1 = A R1^{-1}: the
                 // Gram matrix of the product streamed from A, then the check
@@ -179,24 +195,22 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     if (flat_auto) {
         // FP32 from kStream4MinRows rows: the R pass on 4 x 4 tiles, with its
         // own grid
-        const bool four = sizeof(T) == 4 && use_stream4(d.m);
-        const int rgroups = four ? stream_groups4(d.m) : sgroups;
-        if (four) {
+        if (sizeof(T) == 4 && use_stream4(d.m)) {
             if constexpr (sizeof(T) == 4) {
+                const int rgroups = stream_groups4(d.m);
                 if (d.n == N)
                     hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, true>), dim3(rgroups), dim3(256), 0, st, d.m,
                                        d.n, (const float*)A, lda, partial, stream_tk, kCachedChunks);
                 else
                     hipLaunchKernelGGL((tsqr_gram_stream4<N / 4, VEC, false>), dim3(rgroups), dim3(256), 0, st, d.m,
                                        d.n, (const float*)A, lda, partial, stream_tk, kCachedChunks);
+                hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d,
+                                   stream_group_count(rgroups), (const double*)(partial + (size_t)rgroups * 1024), R,
+                                   ldr, side, ctl, o.gate, (double*)nullptr);
             }
         } else {
-            hipLaunchKernelGGL((tsqr_gram_stream<T, NT, VEC>), dim3(sgroups), dim3(256), 0, st, d, (const T*)A, lda,
-                               partial, stream_tk);
+            gram_and_factor(stream_tk, o.gate, (double*)nullptr, (const int*)nullptr);
         }
-        hipLaunchKernelGGL((tsqr_gram_flat<T>), dim3(1), dim3(kSumThreads), 0, st, d, stream_group_count(rgroups),
-                           (const double*)(partial + (size_t)rgroups * 1024), R, ldr, side, ctl, o.gate,
-                           (double*)nullptr);
     }
     if (gram_tree) {
         const int gram_grid = leaves < kFallbackGrid ? leaves : kFallbackGrid;
