@@ -114,14 +114,18 @@ __device__ __forceinline__ void load8(float (&x)[8], const float* __restrict__ b
     }
 }
 
-// Pass 1a of the batch: the BF16x3 Gram matrix of row block `blk` of S of
-// an m-row panel (one wave), left in the wave's FP64 slab `sums`.  The row
-// blocks partition the panel's 64-row chunks by stride: block j takes
-// chunks j, j + S, j + 2S, ..., so the S waves of a panel sweep it together
-// over a window of S chunks (the Grams are only ever summed).  Lane
-// l loads rows 8(l/16)..+7 of column l%16 (and 16 + l%16) of each 32-row
-// step; a chunk's two steps are loaded together and the next chunk is in
-// flight while the current one is multiplied.
+// Pass 1a, one wave: the BF16x3 Gram matrix of row block `blk` of S of an
+// m-row panel, left in the wave's FP64 slab `sums`.  The row blocks
+// partition the panel's 64-row chunks by stride: block j takes chunks j,
+// j + S, j + 2S, ..., so the S waves of a panel sweep it together over a
+// window of S chunks (the Grams are only ever summed).  Lane l loads rows
+// 8(l/16)..+7 of column l%16 (and 16 + l%16) of each 32-row step.  Full
+// chunks go through two buffers in turn (ping-pong): unconditional loads
+// issued before the previous chunk's products (a branch around a load made
+// the compiler wait for every load in flight), padding columns (n < 32)
+// loaded as column 0 and zeroed where they are used; the short last chunk
+// is loaded with bounds checks.  Used by the batch (one row block per wave
+// of an item) and by tsqr_mixed_stream (the waves of a persistent grid).
 template <int NT, bool VEC>
 __device__ __forceinline__ void mixed_gram_body(int m, int n, const float* __restrict__ A, int lda, double* sums,
                                                 int blk, int S) {
@@ -129,38 +133,68 @@ __device__ __forceinline__ void mixed_gram_body(int m, int n, const float* __res
     const int col = lane & 15;
     const int rq = 8 * (lane >> 4);
     const int chunks = (m + 63) / 64;
-    constexpr int D = 2;  // 32-row steps per chunk (a chunk and its successor in flight: 64 VGPRs per column half)
+    constexpr int D = 2;  // 32-row steps per chunk
     GramAcc32 g;
     g.sums = sums;
     g.zero(lane);
-    auto load = [&](int ch, float(&x0)[D][8], float(&x1)[D][8]) {
-        const float* b = A + (int64_t)ch * 64;
-        const int lim = m - 64 * ch;
+    const bool in0 = col < n, in1 = col + 16 < n;
+    const int c0col = in0 ? col : 0, c1col = in1 ? col + 16 : 0;
+    const int nfull = m / 64;
+    auto load_chunk = [&](int ch, float(&x0)[D][8], float(&x1)[D][8]) {
 #pragma unroll
         for (int q = 0; q < D; ++q) {
-            load8<VEC>(x0[q], b, lda, 32 * q + rq, col, lim, n);
-            if constexpr (NT == 2) load8<VEC>(x1[q], b, lda, 32 * q + rq, col + 16, lim, n);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const Raw4<float> a = load_col4<float, VEC>(A, lda, 64 * ch + 32 * q + rq + 4 * h, c0col);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) x0[q][4 * h + t] = a.v[t];
+                if constexpr (NT == 2) {
+                    const Raw4<float> b = load_col4<float, VEC>(A, lda, 64 * ch + 32 * q + rq + 4 * h, c1col);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) x1[q][4 * h + t] = b.v[t];
+                }
+            }
         }
     };
-    float c0[D][8], c1[D][8];
-    int ch = blk;
-    if (ch < chunks) load(ch, c0, c1);
-#pragma unroll 1
-    for (; ch < chunks; ch += S) {
-        const int nx = ch + S;
-        float p0[D][8], p1[D][8];
-        if (nx < chunks) load(nx, p0, p1);
+    auto use_chunk = [&](float(&x0)[D][8], float(&x1)[D][8]) {
 #pragma unroll
-        for (int q = 0; q < D; ++q) g.add<NT>(c0[q], c1[q]);
-        g.flush<NT>(lane);
-        if (nx >= chunks) break;
+        for (int q = 0; q < D; ++q) {
 #pragma unroll
-        for (int q = 0; q < D; ++q)
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                c0[q][t] = p0[q][t];
-                if constexpr (NT == 2) c1[q][t] = p1[q][t];
+            for (int k = 0; k < 8; ++k) {
+                x0[q][k] = in0 ? x0[q][k] : 0.f;
+                if constexpr (NT == 2) x1[q][k] = in1 ? x1[q][k] : 0.f;
             }
+            g.add<NT>(x0[q], x1[q]);
+        }
+        g.flush<NT>(lane);
+    };
+    if (blk < nfull) {
+        const int nch = (nfull - blk + S - 1) / S;
+        float a0[D][8], a1[D][8], b0[D][8], b1[D][8];
+        load_chunk(blk, a0, a1);
+#pragma unroll 1
+        for (int k = 0; k + 1 < nch; k += 2) {
+            load_chunk(blk + (k + 1) * S, b0, b1);
+            __builtin_amdgcn_sched_barrier(0);
+            use_chunk(a0, a1);
+            __builtin_amdgcn_sched_barrier(0);
+            const int ka = k + 2 < nch ? k + 2 : nch - 1;  // the last pair reloads a chunk it has
+            load_chunk(blk + ka * S, a0, a1);
+            __builtin_amdgcn_sched_barrier(0);
+            use_chunk(b0, b1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (nch & 1) use_chunk(a0, a1);
+    }
+    if (chunks > nfull && blk == nfull % S) {  // the short last chunk, bounds-checked loads
+        float c0[D][8], c1[D][8];
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            load8<VEC>(c0[q], A, lda, 64 * nfull + 32 * q + rq, col, m, n);
+            if constexpr (NT == 2) load8<VEC>(c1[q], A, lda, 64 * nfull + 32 * q + rq, col + 16, m, n);
+            g.add<NT>(c0[q], c1[q]);
+        }
+        g.flush<NT>(lane);
     }
 }
 
@@ -237,82 +271,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
     __shared__ double sums[4][12 * 64];
     __shared__ GramLds L;
     __shared__ int bcast;
-    const int t = threadIdx.x;
-    const int lane = t & 63;
-    const int wv = t >> 6;
-    const int m = d.m, n = d.n;
-    const int W = 4 * (int)gridDim.x;
-    const int w = 4 * blockIdx.x + wv;
-    const int chunks = (m + 63) / 64;
-    const int col = lane & 15;
-    const int rq = 8 * (lane >> 4);
-    constexpr int D = 2;  // 32-row steps per chunk
-    GramAcc32 g;
-    g.sums = sums[wv];
-    g.zero(lane);
-    // Full 64-row chunks in two buffers in turn (ping-pong, as
-    // tsqr_gram_stream): unconditional loads issued before the previous
-    // chunk's products, padding columns (n < 32) loaded as column 0 and
-    // zeroed only where they are used.
-    const bool in0 = col < n, in1 = col + 16 < n;
-    const int c0col = in0 ? col : 0, c1col = in1 ? col + 16 : 0;
-    const int nfull = m / 64;
-    auto load_chunk = [&](int ch, float(&x0)[D][8], float(&x1)[D][8]) {
-#pragma unroll
-        for (int q = 0; q < D; ++q) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const Raw4<float> a = load_col4<float, VEC>(A, lda, 64 * ch + 32 * q + rq + 4 * h, c0col);
-#pragma unroll
-                for (int t = 0; t < 4; ++t) x0[q][4 * h + t] = a.v[t];
-                if constexpr (NT == 2) {
-                    const Raw4<float> b = load_col4<float, VEC>(A, lda, 64 * ch + 32 * q + rq + 4 * h, c1col);
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) x1[q][4 * h + t] = b.v[t];
-                }
-            }
-        }
-    };
-    auto use_chunk = [&](float(&x0)[D][8], float(&x1)[D][8]) {
-#pragma unroll
-        for (int q = 0; q < D; ++q) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                x0[q][k] = in0 ? x0[q][k] : 0.f;
-                if constexpr (NT == 2) x1[q][k] = in1 ? x1[q][k] : 0.f;
-            }
-            g.add<NT>(x0[q], x1[q]);
-        }
-        g.flush<NT>(lane);
-    };
-    if (w < nfull) {
-        const int nch = (nfull - w + W - 1) / W;
-        float a0[D][8], a1[D][8], b0[D][8], b1[D][8];
-        load_chunk(w, a0, a1);
-#pragma unroll 1
-        for (int k = 0; k + 1 < nch; k += 2) {
-            load_chunk(w + (k + 1) * W, b0, b1);
-            __builtin_amdgcn_sched_barrier(0);
-            use_chunk(a0, a1);
-            __builtin_amdgcn_sched_barrier(0);
-            const int ka = k + 2 < nch ? k + 2 : nch - 1;  // the last pair reloads a chunk it has
-            load_chunk(w + ka * W, a0, a1);
-            __builtin_amdgcn_sched_barrier(0);
-            use_chunk(b0, b1);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        if (nch & 1) use_chunk(a0, a1);
-    }
-    if (chunks > nfull && w == nfull % W) {  // the short last chunk, bounds-checked loads
-        float c0[D][8], c1[D][8];
-#pragma unroll
-        for (int q = 0; q < D; ++q) {
-            load8<VEC>(c0[q], A, lda, 64 * nfull + 32 * q + rq, col, m, n);
-            if constexpr (NT == 2) load8<VEC>(c1[q], A, lda, 64 * nfull + 32 * q + rq, col + 16, m, n);
-            g.add<NT>(c0[q], c1[q]);
-        }
-        g.flush<NT>(lane);
-    }
+    const int wv = threadIdx.x >> 6;
+    mixed_gram_body<NT, VEC>(d.m, d.n, A, lda, sums[wv], 4 * (int)blockIdx.x + wv, 4 * (int)gridDim.x);
     gram32_finish<NT>(L, sums, partial, tk, &bcast);
 }
 
