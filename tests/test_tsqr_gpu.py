@@ -134,6 +134,38 @@ def test_short_r_pass_boundary(m, n, dtype, path):
     check(A, Q, R, f"{m}x{n} {np.dtype(dtype).name} path={path}", n_floor=floor)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_short_calls_reuse_one_workspace(dtype):
+    """Short calls set up their workspace inside the R pass's launch: on one
+    workspace, calls that take the flat path and calls whose gate sends them
+    to the Householder fallback alternate, and each starts from a clean
+    set-up (tickets, flags, list and control words of the previous call
+    cleared)."""
+    import torch
+
+    import tsqr_amd
+    from tsqr_amd._lib import TSQR_DOUBLE, TSQR_FLOAT
+    from tests.gpu_util import from_device_colmajor, to_device_colmajor
+
+    m, n = 27648, 32
+    code = TSQR_FLOAT if dtype == np.float32 else TSQR_DOUBLE
+    entry = "tsqr_s_stream" if dtype == np.float32 else "tsqr_d_stream"
+    L = tsqr_amd.lib()
+    need = L.tsqr_workspace_bytes(m, n, code)
+    work = torch.empty(need, dtype=torch.uint8, device="cuda")
+    conds = [None, 1e6, None, 1e6, None] if dtype == np.float32 else [None, 1e12, None]
+    for k, cond in enumerate(conds):
+        A = (O.gen_uniform(m, n, dtype, seed=50 + k) if cond is None
+             else O.gen_conditioned(m, n, cond, dtype, seed=50 + k))
+        buf, pA, lda = to_device_colmajor(A)
+        R = torch.full((n * n,), float("nan"), dtype=buf.dtype, device="cuda")
+        assert getattr(L, entry)(m, n, pA, lda, R.data_ptr(), n, work.data_ptr(), need, None) == 0
+        torch.cuda.synchronize()
+        Q = from_device_colmajor(buf, m, n, lda)
+        floor = 32 if dtype == np.float64 else 1
+        check(A, Q, R.view(n, n).cpu().numpy().T.copy(), f"call {k} cond={cond}", n_floor=floor)
+
+
 @pytest.mark.parametrize("path", [pytest.param(AUTO, id="auto"), pytest.param(CHOLQR2, id="cholqr2")])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("m,n", SHAPES)

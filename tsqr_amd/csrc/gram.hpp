@@ -992,12 +992,17 @@ __device__ __forceinline__ double stream_lds_sum16(GramLds (&Q)[4], const v4d& a
     return ((Q[0].G[r][c] + Q[1].G[r][c]) + Q[2].G[r][c]) + Q[3].G[r][c];
 }
 
-// partial: gridDim.x blocks of 32 x 32 FP64.
+// partial: gridDim.x blocks of 32 x 32 FP64.  ini.on: the grid also sets
+// up the call's workspace (the launch is the call's first, and nothing in it
+// reads the words it writes).
 template <typename T, int NT, bool VEC>
 __global__ __launch_bounds__(1024) void tsqr_gram_stream16(const TreeDesc d, const T* __restrict__ A, int lda,
                                                            double* __restrict__ partial,
-                                                           const int* __restrict__ skip = nullptr) {
+                                                           const int* __restrict__ skip, const WsInit ini) {
     __shared__ GramLds Q[4];
+    if (ini.on)
+        for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < ini.count(); i += (int)(gridDim.x * blockDim.x))
+            ini.at(i);
     if (skip && skip[0]) return;  // an earlier pass already took the flat path
     v4d a00 = {0, 0, 0, 0}, a01 = {0, 0, 0, 0}, a11 = {0, 0, 0, 0};
     gram_stream_acc<T, NT, VEC>(d, A, lda, 16 * (int)gridDim.x, 16 * (int)blockIdx.x + (int)(threadIdx.x >> 6), a00,

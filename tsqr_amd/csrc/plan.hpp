@@ -239,6 +239,34 @@ inline size_t check_workspace_bytes(int64_t m) {
     return 3 * stream_partials(m) * 1024 * sizeof(double) + 3 * kCtlStreamTickets * sizeof(int);
 }
 
+// The set-up of one TSQR call's workspace: tickets = 0, flags = value, the
+// Householder list empty (value 0) or holding every superleaf (value 1), the
+// control words cleared.  Run by init_flags, or folded into the first kernel
+// of a call (tsqr_gram_stream16) where that kernel uses none of these words.
+struct WsInit {
+    int* tickets;
+    int nt;
+    int* flags;
+    int* list;
+    int nf;
+    int value;
+    int* ctl;
+    bool on;  // folded into the kernel that receives it
+    __host__ __device__ int count() const {
+        const int a = nt > nf ? nt : nf;
+        return a > kCtlWords ? a : kCtlWords;
+    }
+    __device__ inline void at(int i) const {
+        if (i < nt) tickets[i] = 0;
+        if (i < nf) {
+            flags[i] = value;
+            if (value) list[1 + i] = i;
+        }
+        if (i < kCtlWords) ctl[i] = 0;
+        if (i == 0) list[0] = value ? nf : 0;
+    }
+};
+
 struct WsLayout {
     size_t blocks, tickets, flags, list, ctl, partial, side, total;
 };

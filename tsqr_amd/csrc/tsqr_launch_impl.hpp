@@ -53,18 +53,10 @@ constexpr int kFusedWalkLeaves = 256;
 constexpr int kFallbackTileGrid = 2048;
 
 namespace {
-// tickets = 0; flags = value; the Householder list is empty (value 0) or
-// holds every superleaf (value 1); the control words are cleared.
-__global__ void init_flags(int* __restrict__ tickets, int nt, int* __restrict__ flags, int* __restrict__ list,
-                           int nf, int value, int* __restrict__ ctl) {
+// The workspace set-up of a call (WsInit), one thread per word.
+__global__ void init_flags(const WsInit w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nt) tickets[i] = 0;
-    if (i < nf) {
-        flags[i] = value;
-        if (value) list[1 + i] = i;
-    }
-    if (i < kCtlWords) ctl[i] = 0;
-    if (i == 0) list[0] = value ? nf : 0;
+    if (i < w.count()) w.at(i);
 }
 
 }  // namespace
@@ -105,9 +97,18 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     const int force = gram_tree ? 0 : 1;
     constexpr int NT = N > 16 ? 2 : 1;
 
-    const int ninit = d.total_ctr > leaves ? d.total_ctr : leaves;
-    hipLaunchKernelGGL(init_flags, dim3((ninit + kCtlWords - 1 + 255) / 256), dim3(256), 0, st, tickets,
-                       d.total_ctr, flags, list, leaves, force, ctl);
+    // Short matrices (use_stream_small): the R pass without tickets on
+    // ssmall workgroups of 16 waves, summed by the flat factor's 8 waves.
+    // Where it is the call's first kernel and reads no control word (the
+    // AUTO flat path, the FP64 CholeskyQR2 path), it also sets up the
+    // workspace, one launch fewer.
+    constexpr int kChunkRows = 16 * stream_slabs<T>();
+    const bool small = use_stream_small(d.m, kChunkRows);
+    const int ssmall = stream_small_groups(d.m, kChunkRows);
+    WsInit ini{tickets, d.total_ctr, flags, list, leaves, force, ctl, false};
+    const bool fold_init = small && (flat_auto || (mixed && sizeof(T) == 8));
+    if (!fold_init) hipLaunchKernelGGL(init_flags, dim3((ini.count() + 255) / 256), dim3(256), 0, st, ini);
+    ini.on = fold_init;
     // Householder superleaf kernels: a grid-stride loop over the list
     const int super_grid = leaves < 1024 ? leaves : 1024;
     double* rkeep = side + (size_t)(d.side_leaves + 1) * 1024;
@@ -118,16 +119,12 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     const int sgroup_parts = stream_group_count(sgroups);
     double* stream_sum = partial + (size_t)sgroups * 1024;
     const double* X0 = side + (size_t)d.side_leaves * 1024;  // R^{-1} of the flat path
-    // Short matrices (use_stream_small): the R pass without tickets on
-    // ssmall workgroups of 16 waves, summed by the flat factor's 8 waves.
-    constexpr int kChunkRows = 16 * stream_slabs<T>();
-    const bool small = use_stream_small(d.m, kChunkRows);
-    const int ssmall = stream_small_groups(d.m, kChunkRows);
     // the R pass on 16 x 16 tiles (FP64 products) and the flat factor
     auto gram_and_factor = [&](int* tk, double gate, double* keep, const int* skip) {
         if (small) {
             hipLaunchKernelGGL((tsqr_gram_stream16<T, NT, VEC>), dim3(ssmall), dim3(1024), 0, st, d, (const T*)A,
-                               lda, partial, skip);
+                               lda, partial, skip, ini);
+            ini.on = false;
             hipLaunchKernelGGL((tsqr_gram_flat16<T>), dim3(1), dim3(kFlat16Threads), 0, st, d, ssmall,
                                (const double*)partial,
                                R, ldr, side, ctl, gate, keep, skip);
