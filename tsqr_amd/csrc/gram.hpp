@@ -944,21 +944,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
 }
 
 // ---------------------------------------------------------------------------
-// Short matrices (plan.hpp, use_stream_small): the R pass on workgroups of
-// 16 waves with one chunk per wave (tsqr_gram_stream16) and no tickets;
-// each workgroup sums its waves in LDS and writes partial[blockIdx.x], and
-// tsqr_gram_flat16 sums the P partials with 8 waves (up to four blocks
-// each, in flight together) and factors.  Measured on MI355X for 27648 x 32: the ticketed
-// 4-wave launch spends about 8 of its 12-15 us after its products (the
-// release fence before each ticket, about 3.5 us, the acquire and the group
-// sums); 4-wave workgroups without tickets leave four times the partials,
-// which one workgroup cannot sum in one round trip.
+// Short matrices (plan.hpp, short_waves): the R pass without tickets on
+// workgroups of W = 4, 8 or 16 waves with one chunk per wave
+// (tsqr_gram_short), the fewest waves that keep the grid within
+// kShortGroups; each workgroup sums its waves in LDS and writes
+// partial[blockIdx.x], and tsqr_flat_short sums the P partials with 8
+// waves (four blocks in flight each) and factors.  Measured on MI355X for
+// 27648 x 32: the ticketed 4-wave launch spends about 8 of its 12-15 us
+// after its products (the release fence before each ticket, about 3.5 us,
+// the acquire and the group sums); on 16-wave workgroups (27 CUs) the FP64
+// products alone take 7.5 us, on 8-wave ones (54 CUs) about half.
 
-// Sums the 16 waves' FP64 MFMA accumulators (stream_lds_sum's layout) in
+// Sums the W waves' FP64 MFMA accumulators (stream_lds_sum's layout) in
 // four LDS blocks (wave w adds into Q[w % 4], in the order w / 4), then
-// returns entry t = threadIdx.x of the total, ((Q0 + Q1) + Q2) + Q3.
-template <int NT>
-__device__ __forceinline__ double stream_lds_sum16(GramLds (&Q)[4], const v4d& a00, const v4d& a01, const v4d& a11) {
+// writes the total, ((Q0 + Q1) + Q2) + Q3, to out (1024 entries).
+template <int NT, int W>
+__device__ __forceinline__ void stream_lds_write(GramLds (&Q)[4], const v4d& a00, const v4d& a01, const v4d& a11,
+                                                 double* __restrict__ out) {
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wv = t >> 6;
@@ -966,7 +968,7 @@ __device__ __forceinline__ double stream_lds_sum16(GramLds (&Q)[4], const v4d& a
     const int i0 = lane >> 4;
     GramLds& L = Q[wv & 3];
 #pragma unroll 1
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < W / 4; ++s) {
         if ((wv >> 2) == s) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -988,26 +990,30 @@ __device__ __forceinline__ double stream_lds_sum16(GramLds (&Q)[4], const v4d& a
         }
         __syncthreads();
     }
-    const int r = t >> 5, c = t & 31;
-    return ((Q[0].G[r][c] + Q[1].G[r][c]) + Q[2].G[r][c]) + Q[3].G[r][c];
+#pragma unroll
+    for (int h = 0; h < 16 / W; ++h) {
+        const int idx = t + 64 * W * h;
+        const int r = idx >> 5, c = idx & 31;
+        out[idx] = ((Q[0].G[r][c] + Q[1].G[r][c]) + Q[2].G[r][c]) + Q[3].G[r][c];
+    }
 }
 
 // partial: gridDim.x blocks of 32 x 32 FP64.  ini.on: the grid also sets
 // up the call's workspace (the launch is the call's first, and nothing in it
 // reads the words it writes).
-template <typename T, int NT, bool VEC>
-__global__ __launch_bounds__(1024) void tsqr_gram_stream16(const TreeDesc d, const T* __restrict__ A, int lda,
-                                                           double* __restrict__ partial,
-                                                           const int* __restrict__ skip, const WsInit ini) {
+template <typename T, int NT, bool VEC, int W>
+__global__ __launch_bounds__(64 * W) void tsqr_gram_short(const TreeDesc d, const T* __restrict__ A, int lda,
+                                                          double* __restrict__ partial,
+                                                          const int* __restrict__ skip, const WsInit ini) {
     __shared__ GramLds Q[4];
     if (ini.on)
         for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < ini.count(); i += (int)(gridDim.x * blockDim.x))
             ini.at(i);
     if (skip && skip[0]) return;  // an earlier pass already took the flat path
     v4d a00 = {0, 0, 0, 0}, a01 = {0, 0, 0, 0}, a11 = {0, 0, 0, 0};
-    gram_stream_acc<T, NT, VEC>(d, A, lda, 16 * (int)gridDim.x, 16 * (int)blockIdx.x + (int)(threadIdx.x >> 6), a00,
+    gram_stream_acc<T, NT, VEC>(d, A, lda, W * (int)gridDim.x, W * (int)blockIdx.x + (int)(threadIdx.x >> 6), a00,
                                 a01, a11);
-    partial[(int64_t)blockIdx.x * 1024 + threadIdx.x] = stream_lds_sum16<NT>(Q, a00, a01, a11);
+    stream_lds_write<NT, W>(Q, a00, a01, a11, partial + (int64_t)blockIdx.x * 1024);
 }
 
 // Q[0].G = the sum of `count` 32 x 32 FP64 blocks at src by 8 waves (512
@@ -1015,7 +1021,7 @@ __global__ __launch_bounds__(1024) void tsqr_gram_stream16(const TreeDesc d, con
 // in flight per step, then the wave sums are added in four LDS blocks (wave
 // w into Q[w % 4], in the order w / 4) and those as ((Q0 + Q1) + Q2) + Q3.
 // Every thread of the workgroup must call it; Q[0] is synchronised on return.
-constexpr int kFlat16Threads = 512;
+constexpr int kFlatShortThreads = 512;
 __device__ __forceinline__ void sum_blocks_wg8(GramLds (&Q)[4], const double* __restrict__ src, int count) {
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -1055,24 +1061,24 @@ __device__ __forceinline__ void sum_blocks_wg8(GramLds (&Q)[4], const double* __
     double sum[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const int idx = t + kFlat16Threads * h;
+        const int idx = t + kFlatShortThreads * h;
         const int r = idx >> 5, c = idx & 31;
         sum[h] = ((Q[0].G[r][c] + Q[1].G[r][c]) + Q[2].G[r][c]) + Q[3].G[r][c];
     }
     __syncthreads();
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const int idx = t + kFlat16Threads * h;
+        const int idx = t + kFlatShortThreads * h;
         Q[0].G[idx >> 5][idx & 31] = sum[h];
     }
     __syncthreads();
 }
 
 // Flat factor of a short matrix (one workgroup of 8 waves): G = the sum of
-// the nparts partials of tsqr_gram_stream16, then flat_factor as
+// the nparts partials of tsqr_gram_short, then flat_factor as
 // tsqr_gram_flat.
 template <typename T>
-__global__ __launch_bounds__(kFlat16Threads) void tsqr_gram_flat16(const TreeDesc d, int nparts,
+__global__ __launch_bounds__(kFlatShortThreads) void tsqr_flat_short(const TreeDesc d, int nparts,
                                                          const double* __restrict__ partial, T* __restrict__ R,
                                                          int ldr, double* __restrict__ side, int* __restrict__ ctl,
                                                          double gate, double* __restrict__ rkeep,

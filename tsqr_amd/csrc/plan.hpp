@@ -176,20 +176,24 @@ inline int stream_groups(int64_t m) {
     return (int)(p < cap ? p : cap);
 }
 
-// Short matrices: the R pass without tickets on workgroups of 16 waves, one
-// chunk of chunk_rows rows per wave (tsqr_gram_stream16), at most
-// kStreamSmallGroups workgroups, whose partials tsqr_gram_flat16 sums with
-// 16 waves.  A ticketed launch spends about 8 us of a 27648 x 32 call on the
-// release fence, the ticket and the group sums.  The workspace holds the
-// partials: stream_groups(m) >= stream_small_groups(m, .) here.
-constexpr int kStreamSmallGroups = 64;
-inline int stream_small_groups(int64_t m, int chunk_rows) {
+// Short matrices: the R pass without tickets, one chunk of chunk_rows rows
+// (64 FP32, 32 FP64) per wave, on workgroups of the fewest waves (4, 8 or
+// 16) that keep the grid within kShortGroups (tsqr_gram_short); their
+// partials tsqr_flat_short sums with 8 waves.  A ticketed launch spends
+// about 8 us of a 27648 x 32 call on the release fence, the ticket and the
+// group sums; fewer, wider workgroups leave fewer partials but put the FP64
+// products on fewer CUs.  short_waves is 0 above 16 kShortGroups chunks.
+constexpr int kShortGroups = 64;
+inline int short_waves(int64_t m, int chunk_rows) {
     const int64_t chunks = (m + chunk_rows - 1) / chunk_rows;
-    const int64_t p = (chunks + 15) / 16;
-    return (int)(p < 1 ? 1 : (p > kStreamSmallGroups + 1 ? kStreamSmallGroups + 1 : p));
+    for (int w = 4; w <= 16; w *= 2)
+        if ((chunks + w - 1) / w <= kShortGroups) return w;
+    return 0;
 }
-inline bool use_stream_small(int64_t m, int chunk_rows) {
-    return stream_small_groups(m, chunk_rows) <= kStreamSmallGroups;
+inline int short_groups(int64_t m, int chunk_rows) {
+    const int w = short_waves(m, chunk_rows);
+    const int64_t chunks = (m + chunk_rows - 1) / chunk_rows;
+    return w ? (int)((chunks + w - 1) / w) : 0;
 }
 
 // The FP32 flat path's passes on 4 x 4 tiles (stream4.hpp).  The R pass
@@ -242,7 +246,7 @@ inline size_t check_workspace_bytes(int64_t m) {
 // The set-up of one TSQR call's workspace: tickets = 0, flags = value, the
 // Householder list empty (value 0) or holding every superleaf (value 1), the
 // control words cleared.  Run by init_flags, or folded into the first kernel
-// of a call (tsqr_gram_stream16) where that kernel uses none of these words.
+// of a call (tsqr_gram_short) where that kernel uses none of these words.
 struct WsInit {
     int* tickets;
     int nt;
@@ -283,7 +287,11 @@ inline WsLayout ws_layout(const TreeDesc& d, size_t elem) {
     const size_t sl = (size_t)d.side_leaves;
     const size_t parts = sl < (size_t)kGramPartials ? sl : (size_t)kGramPartials;
     const size_t parts2 = parts > (size_t)kGramPartials2 ? (size_t)kGramPartials2 : 0;
-    const size_t sparts = stream_partials(d.m);
+    // the streaming passes' partials, or the short R pass's (chunks of 64
+    // FP32 or 32 FP64 rows)
+    size_t sparts = stream_partials(d.m);
+    const size_t shorts = (size_t)short_groups(d.m, elem == 4 ? 64 : 32);
+    if (shorts > sparts) sparts = shorts;
     const size_t nparts = parts + parts2 > sparts ? parts + parts2 : sparts;
     w.side = w.partial + nparts * 1024 * sizeof(double);
     w.total = w.side + align256((sl + 3) * 1024 * sizeof(double));

@@ -97,14 +97,15 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     const int force = gram_tree ? 0 : 1;
     constexpr int NT = N > 16 ? 2 : 1;
 
-    // Short matrices (use_stream_small): the R pass without tickets on
-    // ssmall workgroups of 16 waves, summed by the flat factor's 8 waves.
+    // Short matrices (short_waves): the R pass without tickets on ssmall
+    // workgroups of swaves waves, summed by the flat factor's 8 waves.
     // Where it is the call's first kernel and reads no control word (the
     // AUTO flat path, the FP64 CholeskyQR2 path), it also sets up the
     // workspace, one launch fewer.
     constexpr int kChunkRows = 16 * stream_slabs<T>();
-    const bool small = use_stream_small(d.m, kChunkRows);
-    const int ssmall = stream_small_groups(d.m, kChunkRows);
+    const int swaves = short_waves(d.m, kChunkRows);
+    const bool small = swaves != 0;
+    const int ssmall = short_groups(d.m, kChunkRows);
     WsInit ini{tickets, d.total_ctr, flags, list, leaves, force, ctl, false};
     const bool fold_init = small && (flat_auto || (mixed && sizeof(T) == 8));
     if (!fold_init) hipLaunchKernelGGL(init_flags, dim3((ini.count() + 255) / 256), dim3(256), 0, st, ini);
@@ -122,10 +123,17 @@ int launch_tsqr_n(const TreeDesc& d, T* A, int lda, T* R, int ldr, void* work, c
     // the R pass on 16 x 16 tiles (FP64 products) and the flat factor
     auto gram_and_factor = [&](int* tk, double gate, double* keep, const int* skip) {
         if (small) {
-            hipLaunchKernelGGL((tsqr_gram_stream16<T, NT, VEC>), dim3(ssmall), dim3(1024), 0, st, d, (const T*)A,
-                               lda, partial, skip, ini);
+            if (swaves == 4)
+                hipLaunchKernelGGL((tsqr_gram_short<T, NT, VEC, 4>), dim3(ssmall), dim3(256), 0, st, d, (const T*)A,
+                                   lda, partial, skip, ini);
+            else if (swaves == 8)
+                hipLaunchKernelGGL((tsqr_gram_short<T, NT, VEC, 8>), dim3(ssmall), dim3(512), 0, st, d, (const T*)A,
+                                   lda, partial, skip, ini);
+            else
+                hipLaunchKernelGGL((tsqr_gram_short<T, NT, VEC, 16>), dim3(ssmall), dim3(1024), 0, st, d,
+                                   (const T*)A, lda, partial, skip, ini);
             ini.on = false;
-            hipLaunchKernelGGL((tsqr_gram_flat16<T>), dim3(1), dim3(kFlat16Threads), 0, st, d, ssmall,
+            hipLaunchKernelGGL((tsqr_flat_short<T>), dim3(1), dim3(kFlatShortThreads), 0, st, d, ssmall,
                                (const double*)partial,
                                R, ldr, side, ctl, gate, keep, skip);
         } else {
