@@ -948,16 +948,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
 // workgroups of W = 4, 8 or 16 waves with one chunk per wave
 // (tsqr_gram_short), the fewest waves that keep the grid within
 // kShortGroups; each workgroup sums its waves in LDS and writes
-// partial[blockIdx.x], and tsqr_flat_short sums the P partials with 8
-// waves (four blocks in flight each) and factors.  Measured on MI355X for
+// partial[blockIdx.x] (its three distinct 16 x 16 blocks), and
+// tsqr_flat_short sums the P partials with 8 waves (all in flight at once)
+// and factors.  Measured on MI355X for
 // 27648 x 32: the ticketed 4-wave launch spends about 8 of its 12-15 us
 // after its products (the release fence before each ticket, about 3.5 us,
 // the acquire and the group sums); on 16-wave workgroups (27 CUs) the FP64
 // products alone take 7.5 us, on 8-wave ones (54 CUs) about half.
 
+// A short partial holds the three distinct 16 x 16 blocks of a symmetric
+// 32 x 32 Gram matrix, (0,0), (0,1) and (1,1), row-major one after the
+// other: three quarters of the bytes the flat sum reads from other CUs.
+constexpr int kShortPartial = 768;
+
 // Sums the W waves' FP64 MFMA accumulators (stream_lds_sum's layout) in
 // four LDS blocks (wave w adds into Q[w % 4], in the order w / 4), then
-// writes the total, ((Q0 + Q1) + Q2) + Q3, to out (1024 entries).
+// writes the total, ((Q0 + Q1) + Q2) + Q3, to out as a short partial.
 template <int NT, int W>
 __device__ __forceinline__ void stream_lds_write(GramLds (&Q)[4], const v4d& a00, const v4d& a01, const v4d& a11,
                                                  double* __restrict__ out) {
@@ -991,10 +997,13 @@ __device__ __forceinline__ void stream_lds_write(GramLds (&Q)[4], const v4d& a00
         __syncthreads();
     }
 #pragma unroll
-    for (int h = 0; h < 16 / W; ++h) {
+    for (int h = 0; h < (kShortPartial + 64 * W - 1) / (64 * W); ++h) {
         const int idx = t + 64 * W * h;
-        const int r = idx >> 5, c = idx & 31;
-        out[idx] = ((Q[0].G[r][c] + Q[1].G[r][c]) + Q[2].G[r][c]) + Q[3].G[r][c];
+        if (idx < kShortPartial) {
+            const int blk = idx >> 8, i = (idx >> 4) & 15, j = idx & 15;
+            const int r = (blk == 2 ? 16 : 0) + i, c = (blk == 0 ? 0 : 16) + j;
+            out[idx] = ((Q[0].G[r][c] + Q[1].G[r][c]) + Q[2].G[r][c]) + Q[3].G[r][c];
+        }
     }
 }
 
@@ -1016,42 +1025,48 @@ __global__ __launch_bounds__(64 * W) void tsqr_gram_short(const TreeDesc d, cons
     stream_lds_write<NT, W>(Q, a00, a01, a11, partial + (int64_t)blockIdx.x * 1024);
 }
 
-// Q[0].G = the sum of `count` 32 x 32 FP64 blocks at src by 8 waves (512
-// threads): wave w adds the blocks [w c / 8, (w + 1) c / 8) in order, four
-// in flight per step, then the wave sums are added in four LDS blocks (wave
-// w into Q[w % 4], in the order w / 4) and those as ((Q0 + Q1) + Q2) + Q3.
-// Every thread of the workgroup must call it; Q[0] is synchronised on return.
+// Q[0].G = the sum of `count` short partials at src (kShortPartial entries
+// each, one per 1024-double slot) by 8 waves (512 threads): wave w adds the
+// partials [w c / 8, (w + 1) c / 8) in order, all of them in flight at once
+// for c <= 64 (eight per wave), then the wave sums are added in four LDS
+// blocks (wave w into Q[w % 4], in the order w / 4) and those as
+// ((Q0 + Q1) + Q2) + Q3, and the symmetric 32 x 32 matrix is filled in.
+// Every thread of the workgroup must call it; Q[0] is synchronised on
+// return.
 constexpr int kFlatShortThreads = 512;
-__device__ __forceinline__ void sum_blocks_wg8(GramLds (&Q)[4], const double* __restrict__ src, int count) {
+__device__ __forceinline__ void sum_short_wg8(GramLds (&Q)[4], const double* __restrict__ src, int count) {
+    constexpr int E = kShortPartial / 64;  // entries per lane
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int w = t >> 6;
     const int b0 = count * w / 8, b1 = count * (w + 1) / 8;
-    double s[16];
+    double s[E];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) s[e] = 0.0;
-    // four blocks in flight per step, the last step's too (its loads past b1
-    // reread block b1 - 1 and add zero)
+    for (int e = 0; e < E; ++e) s[e] = 0.0;
+    // eight partials in flight per step, the last step's too (its loads past
+    // b1 reread partial b1 - 1 and add zero)
 #pragma unroll 1
-    for (int b = b0; b < b1; b += 4) {
-        double v[4][16];
+    for (int b = b0; b < b1; b += 8) {
+        double v[8][E];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < 8; ++q) {
             const int bq = b + q < b1 ? b + q : b1 - 1;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) v[q][e] = src[(int64_t)bq * 1024 + lane + 64 * e];
+            for (int e = 0; e < E; ++e) v[q][e] = src[(int64_t)bq * 1024 + lane + 64 * e];
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 8; ++q)
 #pragma unroll
-            for (int e = 0; e < 16; ++e) s[e] += b + q < b1 ? v[q][e] : 0.0;
+            for (int e = 0; e < E; ++e) s[e] += b + q < b1 ? v[q][e] : 0.0;
     }
+    // the four LDS blocks hold the partial layout flat (entry idx at
+    // G[idx / 32][idx % 32])
     GramLds& L = Q[w & 3];
 #pragma unroll 1
     for (int k = 0; k < 2; ++k) {
         if ((w >> 2) == k) {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
+            for (int e = 0; e < E; ++e) {
                 const int idx = lane + 64 * e;
                 L.G[idx >> 5][idx & 31] = (k == 0 ? 0.0 : L.G[idx >> 5][idx & 31]) + s[e];
             }
@@ -1063,13 +1078,23 @@ __device__ __forceinline__ void sum_blocks_wg8(GramLds (&Q)[4], const double* __
     for (int h = 0; h < 2; ++h) {
         const int idx = t + kFlatShortThreads * h;
         const int r = idx >> 5, c = idx & 31;
-        sum[h] = ((Q[0].G[r][c] + Q[1].G[r][c]) + Q[2].G[r][c]) + Q[3].G[r][c];
+        sum[h] = idx < kShortPartial ? ((Q[0].G[r][c] + Q[1].G[r][c]) + Q[2].G[r][c]) + Q[3].G[r][c] : 0.0;
     }
     __syncthreads();
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int idx = t + kFlatShortThreads * h;
-        Q[0].G[idx >> 5][idx & 31] = sum[h];
+        if (idx < kShortPartial) {
+            const int blk = idx >> 8, i = (idx >> 4) & 15, j = idx & 15;
+            if (blk == 0) {
+                Q[0].G[i][j] = sum[h];
+            } else if (blk == 1) {
+                Q[0].G[i][16 + j] = sum[h];
+                Q[0].G[16 + j][i] = sum[h];
+            } else {
+                Q[0].G[16 + i][16 + j] = sum[h];
+            }
+        }
     }
     __syncthreads();
 }
@@ -1085,7 +1110,7 @@ __global__ __launch_bounds__(kFlatShortThreads) void tsqr_flat_short(const TreeD
                                                          const int* __restrict__ skip = nullptr) {
     __shared__ GramLds Q[4];
     if (skip && skip[0]) return;  // an earlier pass already took the flat path
-    sum_blocks_wg8(Q, partial, nparts);
+    sum_short_wg8(Q, partial, nparts);
     if (threadIdx.x >= 64) return;
     flat_factor<T>(Q[0], d.n, R, ldr, side + (int64_t)d.side_leaves * 1024, ctl, gate, rkeep, threadIdx.x);
 }
